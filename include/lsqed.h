@@ -1,0 +1,191 @@
+/*
+ * lsqed.h -- C-ABI of the MI355X matrix-free Sz-sector Hamiltonian and
+ * Krylov eigensolver (liblsqed.so).
+ *
+ * The boundary replaces the PETSc matrix that LSQuantumED builds and hands to
+ * SLEPc: `Hamiltonian<L>::hamilt` (include/hamiltonian.hpp:187), created as
+ * MATMPIAIJ in `MatInit` (include/hamiltonian.tmpl_func.hpp:8-56) and passed
+ * to `Solver::SolverInit` (src/solver.cpp:7-24).  A PETSc MATSHELL whose
+ * MATOP_MULT calls lsqed_hv() is the drop-in on the reference side; the
+ * binding is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *  - Every call returns an int status: LSQED_OK (0) or a negative LSQED_ERR_*
+ *    code (mirrors PetscErrorCode); lsqed_last_error() gives the message of
+ *    the last failure on the calling thread.
+ *  - Device buffers are caller-owned, FP64, allocated on the current HIP
+ *    device; vectors are dense in basis-rank (colex) order.
+ *  - `stream` may be NULL (the legacy default stream); kernels are enqueued
+ *    asynchronously on it.  Calls that return host scalars synchronise it.
+ *  - States are bit strings (bit i = spin i up), ranks are colex combinadic
+ *    indices r(s) = sum_k C(p_k, k) as in HamiltHelper::search_index
+ *    (src/hamiltonian.cpp:9-27).  N <= 62.
+ */
+#ifndef LSQED_H
+#define LSQED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* lsqed_stream_t; /* == hipStream_t */
+
+enum {
+  LSQED_OK = 0,
+  LSQED_ERR_ARG = -1,     /* invalid argument (std::invalid_argument in the reference) */
+  LSQED_ERR_HIP = -2,     /* a HIP runtime call failed */
+  LSQED_ERR_OOM = -3,     /* device memory plan does not fit */
+  LSQED_ERR_NOCONV = -4,  /* reserved */
+  LSQED_ERR_STATE = -5    /* handle used in the wrong state */
+};
+
+/* Lattice kinds of src/lattice.cpp. */
+enum { LSQED_CHAIN1D = 0, LSQED_SQUARE2D = 1, LSQED_HONEYCOMB2D = 2 };
+
+/* Build flags: which of the reference's build calls have been made.
+ *  DIAG     -> Hamiltonian::build_diag      (include/hamiltonian.tmpl_func.hpp:129-225)
+ *  OFFDIAG  -> Hamiltonian::build_off_diag  (tmpl:229-344)
+ *  DISORDER -> Hamiltonian::build_disorder  (tmpl:97-125; needs fields) */
+enum { LSQED_BUILD_DIAG = 1, LSQED_BUILD_OFFDIAG = 2, LSQED_BUILD_DISORDER = 4 };
+
+typedef struct lsqed_ham lsqed_ham;
+
+/* Ordered forward-neighbour bonds of one kind (0 = nn, 1 = nnn) as
+ * chain1D/square2D/honeycomb2D::get_nn/get_nnn list them (src/lattice.cpp:
+ * 27-53, 86-136, 172-264), flattened in the loop order of
+ * HamiltHelper::get_coup_elems (tmpl:350-387).  bi/bj may be NULL to query
+ * the count.  Errors on lx*ly != N (lattice.cpp:67-68) and on odd honeycomb
+ * sides (lattice.cpp:153-154). */
+int lsqed_lattice_bonds(int lattice, int nspins, int lx, int ly, int kind, int* bi, int* bj, int* count);
+
+/* Sector size C(N, nup) (Basis::compute_size, src/basis.cpp:38-46). */
+int64_t lsqed_sector_dim(int nspins, int nup);
+
+typedef struct {
+  int nspins;         /* N, even and > 0 (src/environment.cpp:31-35) */
+  int nup;            /* up spins = total_mag + N/2 (src/basis.cpp:66-71) */
+  int n_nn, n_nnn;    /* bond counts */
+  const int* nn_i;    /* host arrays, reference emission order */
+  const int* nn_j;
+  const int* nnn_i;
+  const int* nnn_j;
+  double J1, D1, J2, D2; /* Hamiltonian<L>(env, basis, lattice, J1, D1, J2, D2), tmpl:69-91 */
+} lsqed_params;
+
+/* Replaces Hamiltonian<L>::Hamiltonian + MatInit + prealloc (tmpl:8-91,
+ * :432-531): records the model and uploads the rank tables.  Nothing of
+ * size D is stored.  Build flags start at DIAG|OFFDIAG. */
+int lsqed_ham_create(const lsqed_params* p, lsqed_ham** out);
+int lsqed_ham_destroy(lsqed_ham* h);
+int64_t lsqed_ham_dim(const lsqed_ham* h);
+int lsqed_ham_nspins(const lsqed_ham* h);
+int lsqed_ham_nup(const lsqed_ham* h);
+
+/* Which build calls are in effect (LSQED_BUILD_* bits). */
+int lsqed_ham_set_build(lsqed_ham* h, int flags);
+
+/* Hamiltonian<L>::build_disorder(env, hi) (tmpl:97-125): records the N
+ * longitudinal fields (host array, copied) and sets LSQED_BUILD_DISORDER;
+ * the diagonal becomes D1/4*c_nn + D2/4*c_nnn + 1/2*sum_i(+-h_i).
+ * hi == NULL clears the fields and the flag. */
+int lsqed_ham_set_fields(lsqed_ham* h, const double* hi);
+
+/* y = H x (the MatMult of EPSSolve, src/solver.cpp:45).  x and y: D doubles
+ * on the device, must not alias. */
+int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
+
+/* Diagonal of H exactly as the reference stores it (MatGetDiagonal,
+ * tests/unit_tests/build_diag/chain_1d/test_build_diag60.cpp:29): entries the
+ * reference never sets are 0.  d: D doubles (device). */
+int lsqed_diag(lsqed_ham* h, double* d, lsqed_stream_t stream);
+
+/* Column lists of rows [row0, row0+nrows) for one bond kind, in the
+ * emission order of HamiltHelper::get_coup_elems (tmpl:350-387,
+ * src/hamiltonian.cpp:72-99): cols[r*max_cols + c], ncol[r].  Device
+ * buffers.  max_cols >= number of bonds of that kind. */
+int lsqed_export_cols(lsqed_ham* h, int kind, int64_t row0, int64_t nrows, int max_cols, int64_t* cols,
+                      int32_t* ncol, lsqed_stream_t stream);
+
+/* Unrank rows [row0, row0+n) to states (BasisHelper::search_elem +
+ * Basis::generate_int_basis, src/basis.cpp:9-36, :192-215) and rank states
+ * (HamiltHelper::search_index, src/hamiltonian.cpp:9-27).  Device buffers. */
+int lsqed_unrank(lsqed_ham* h, int64_t row0, int64_t n, uint64_t* states, lsqed_stream_t stream);
+int lsqed_rank(lsqed_ham* h, const uint64_t* states, int64_t n, int64_t* ranks, lsqed_stream_t stream);
+
+/* ---------------------------------------------------------------- solver -- */
+/* EPS settings of Solver::SolverInit (src/solver.cpp:7-24): HEP,
+ * Krylov-Schur, smallest real; EPSSetDimensions(nev, ncv, mpd).  Values <= 0
+ * select the SLEPc defaults: ncv = min(D, max(2 nev, nev+15)), mpd = ncv,
+ * tol = 1e-8, maxit = max(100, 2 D / ncv). */
+typedef struct {
+  int nev, ncv, mpd;
+  double tol;
+  int maxit;
+  uint64_t seed;  /* start vector: splitmix64 hash of (seed, row), uniform in [-1, 1) */
+} lsqed_eps_opts;
+
+typedef struct lsqed_eps lsqed_eps;
+
+/* Solver::SolverInit + Solver::solve (src/solver.cpp:7-52): device-resident
+ * thick-restart Lanczos (Krylov-Schur for a symmetric operator) with full
+ * CGS2 reorthogonalisation; convergence when ||r_i|| / |theta_i| < tol
+ * (EPS_CONV_REL).  Allocates (ncv+1) D-vectors of workspace. */
+int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out);
+int lsqed_eps_solve(lsqed_eps* e, int* nconv, lsqed_stream_t stream);
+/* Solver::solution (src/solver.cpp:58-68): eigenpair i (ascending) and the
+ * relative error ||H x - lambda x|| / |lambda| (EPS_ERROR_RELATIVE).  x may
+ * be NULL; otherwise D doubles on the device (unit norm). */
+int lsqed_eps_solution(lsqed_eps* e, int i, double* lambda, double* x, double* error, lsqed_stream_t stream);
+int lsqed_eps_iterations(const lsqed_eps* e);
+int lsqed_eps_destroy(lsqed_eps* e); /* Solver::SolverClean (src/solver.cpp:28-32) */
+
+/* Plain three-vector Lanczos for the ground energy only (no
+ * reorthogonalisation, no eigenvector): for sectors where the (ncv+1)-vector
+ * workspace does not fit (N=36).  work: 3*D doubles on the device.
+ * Stops when the lowest Ritz value changes by < tol relative over 5 steps
+ * or after maxit steps. */
+int lsqed_lanczos_ground(lsqed_ham* h, int maxit, double tol, uint64_t seed, double* work, double* e0, int* steps,
+                         lsqed_stream_t stream);
+
+/* One three-term Lanczos step (the unit of the steps/s metric), on
+ * unnormalised vectors carried with scale factors (true v = sq*q,
+ * true v_prev = sp*p):
+ *   y = H q;  alpha = sq^2 (q.y);  p <- sq*y - alpha*sq*q - beta*sp*p;
+ *   beta' = ||p||.  The next Lanczos vector is p / beta'.
+ * Traffic: one H.v (with the alpha dot fused) plus one pass reading y, q, p
+ * and writing p.  q, p, y: D doubles on the device. */
+int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta, double* y,
+                       double* alpha, double* beta_next, lsqed_stream_t stream);
+
+/* ----------------------------------------------------------- observables -- */
+/* All pairs <S^z_i S^z_j> (SzSz::ExpectVal, src/spinOperators/correlation.cpp
+ * :86-139), <S^+_i S^-_j> (SpSm, :154-224) for a real state psi (device,
+ * D doubles).  out: N*N doubles on the host, row-major [i*N + j]. */
+int lsqed_obs_szsz(lsqed_ham* h, const double* psi, double* out, lsqed_stream_t stream);
+int lsqed_obs_spsm(lsqed_ham* h, const double* psi, double* out, lsqed_stream_t stream);
+/* <S^z_i> for all i: out N doubles (host). */
+int lsqed_obs_sz(lsqed_ham* h, const double* psi, double* out, lsqed_stream_t stream);
+/* DSF amplitudes a[n*N + i] = <n| S_i |0> as Phys::DynStructFactor forms them
+ * (include/phys.tmpl_func.hpp:166-186) with Sqz (0.5*(+-1), same sector,
+ * spinOPft.tmpl_func.hpp:16-60) when h1 == h0, or Sqp (S^+_i into the sector
+ * of h1 with one more up spin, :77-143).  psi0: D0 doubles, evecs: nev
+ * column-major D1-vectors (device).  out: nev*N doubles (host). */
+int lsqed_obs_dsf(lsqed_ham* h0, lsqed_ham* h1, const double* psi0, const double* evecs, int nev, double* out,
+                  lsqed_stream_t stream);
+
+/* ---------------------------------------------------------------- misc ---- */
+/* Tools::RandomDisorder (src/tools/random_disorder.cpp:33-59): the
+ * reproducible seed list (mt19937_64(1) -> uniform_int(0,1000)) and the
+ * fields h_i = uniform_real(min, max) from mt19937_64(seed). */
+int lsqed_rd_seeds(int rep, int64_t* seeds);
+int lsqed_rd_fields(int64_t seed, int nspins, double lo, double hi, double* out);
+
+const char* lsqed_last_error(void);
+const char* lsqed_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSQED_H */

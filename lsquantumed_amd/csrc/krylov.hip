@@ -1,0 +1,654 @@
+// krylov.hip -- device-resident Krylov-Schur (thick-restart Lanczos) and the
+// plain three-term Lanczos on MI355X.
+//
+// Replaces SLEPc's EPSKRYLOVSCHUR for EPS_HEP / EPS_SMALLEST_REAL as
+// configured by Solver::SolverInit (src/solver.cpp:7-24).  All D-sized data
+// (the ncv+1 basis vectors) stays in HBM; only the projected (ncv x ncv)
+// problem and O(ncv) scalars cross to the host per step.
+//
+// Vector kernels are HBM-streaming: coalesced FP64 loads, wavefront shuffle
+// reductions, per-block partial sums reduced in a fixed order (results are
+// deterministic run to run).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lsqed_internal.hpp"
+
+namespace lsqed {
+
+int hv_grid(const lsqed_ham* h, int64_t ntiles);
+
+constexpr int kThreads = 256;
+constexpr int kRedBlocks = 1024;  // partial sums per reduction
+constexpr int kMaxNcv = 64;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sum over a 256-thread block, valid in thread 0.
+__device__ __forceinline__ double bsum(double v) {
+  __shared__ double red[8];
+  v = wsum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+  return t;
+}
+
+__device__ __forceinline__ void block_range(int64_t n, int64_t& a, int64_t& b) {
+  const int64_t G = gridDim.x, g = blockIdx.x;
+  a = g * n / G;
+  b = (g + 1) * n / G;
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// v[r] = uniform [-1, 1) from splitmix64(seed, r).
+__global__ void k_fill_random(double* __restrict__ v, int64_t n, uint64_t seed) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = splitmix64(seed * 0xD1B54A32D192ED03ull + (uint64_t)r);
+    v[r] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+// part[blockIdx] = sum over the block's range of x*y
+__global__ void k_dot(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                      double* __restrict__ part) {
+  int64_t a, b;
+  block_range(n, a, b);
+  double acc = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) acc += x[r] * y[r];
+  double t = bsum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out[c] = sum_i part[c*n + i]  (one block per column, fixed order)
+__global__ void k_reduce_cols(const double* __restrict__ part, int n, double* __restrict__ out) {
+  const double* p = part + (int64_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += p[i];
+  double t = bsum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// h_j partials: part[j*G + block] = sum_r V[j*ld + r] * w[r] for j in the
+// block's column group (blockIdx.y), 16 columns per group.
+__global__ __launch_bounds__(kThreads) void k_multidot(const double* __restrict__ V, int64_t ld, int m,
+                                                       const double* __restrict__ w, int64_t n,
+                                                       double* __restrict__ part) {
+  int64_t a, b;
+  block_range(n, a, b);
+  const int j0 = blockIdx.y * 16;
+  double acc[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) acc[jj] = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+    const double wr = w[r];
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj)
+      if (j0 + jj < m) acc[jj] += V[(int64_t)(j0 + jj) * ld + r] * wr;
+  }
+  __shared__ double red[4][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) {
+    double t = wsum(acc[jj]);
+    if (lane == 0) red[wid][jj] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16 && j0 + (int)threadIdx.x < m) {
+    const int jj = threadIdx.x;
+    part[(int64_t)(j0 + jj) * gridDim.x + blockIdx.x] = red[0][jj] + red[1][jj] + red[2][jj] + red[3][jj];
+  }
+}
+
+// w -= V[:, 0:m] h ; part[block] = partial ||w||^2
+__global__ __launch_bounds__(kThreads) void k_update(const double* __restrict__ V, int64_t ld, int m,
+                                                     const double* __restrict__ hcoef, double* __restrict__ w,
+                                                     int64_t n, double* __restrict__ part) {
+  __shared__ double hs[kMaxNcv + 1];
+  for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = hcoef[j];
+  __syncthreads();
+  int64_t a, b;
+  block_range(n, a, b);
+  double acc = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < m; ++j) s += V[(int64_t)j * ld + r] * hs[j];
+    const double v = w[r] - s;
+    w[r] = v;
+    acc += v * v;
+  }
+  double t = bsum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_scale(double* __restrict__ v, int64_t n, double a) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    v[r] *= a;
+}
+
+// In-place V[:, 0:k] = V[:, 0:m] S (S column-major m x k), row by row.
+template <int MAXM>
+__global__ __launch_bounds__(kThreads) void k_restart(double* __restrict__ V, int64_t ld, int m, int k,
+                                                      const double* __restrict__ S, int64_t n) {
+  __shared__ double Ss[MAXM * MAXM];
+  for (int i = threadIdx.x; i < m * k; i += blockDim.x) Ss[i] = S[i];
+  __syncthreads();
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double vin[MAXM];
+#pragma unroll
+    for (int j = 0; j < MAXM; ++j) vin[j] = (j < m) ? V[(int64_t)j * ld + r] : 0.0;
+    for (int c = 0; c < k; ++c) {
+      double o = 0.0;
+#pragma unroll
+      for (int j = 0; j < MAXM; ++j)
+        if (j < m) o += vin[j] * Ss[c * m + j];
+      V[(int64_t)c * ld + r] = o;
+    }
+  }
+}
+
+// Three-term Lanczos update: p = a*y + b*q + c*p ; part = partial ||p||^2
+__global__ __launch_bounds__(kThreads) void k_lanczos_update(const double* __restrict__ y,
+                                                             const double* __restrict__ q, double* __restrict__ p,
+                                                             int64_t n, double a, double b, double c,
+                                                             double* __restrict__ part) {
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  double acc = 0.0;
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    const double v = a * y[r] + b * q[r] + c * p[r];
+    p[r] = v;
+    acc += v * v;
+  }
+  double t = bsum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// part = partial ||y - lam x||^2
+__global__ void k_resid(const double* __restrict__ y, const double* __restrict__ x, double lam, int64_t n,
+                        double* __restrict__ part) {
+  int64_t a, b;
+  block_range(n, a, b);
+  double acc = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+    const double v = y[r] - lam * x[r];
+    acc += v * v;
+  }
+  double t = bsum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+static int red_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + 255) / 256)); }
+static int stream_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256)); }
+
+// Sum `ncol` columns of partials (n each) to host.
+static int finish_reduce(lsqed_ham* h, const double* part, int n, int ncol, double* dev_out, double* host_out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_cols, dim3(ncol), dim3(kThreads), 0, s, part, n, dev_out);
+  LSQED_HIP(hipGetLastError());
+  LSQED_HIP(hipMemcpyAsync(host_out, dev_out, ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+  LSQED_HIP(hipStreamSynchronize(s));
+  return LSQED_OK;
+}
+
+// ------------------------------------------------- small dense eigensolver --
+// Cyclic Jacobi for a symmetric n x n matrix A (row-major, destroyed):
+// eigenvalues ascending in w, eigenvectors as columns of Z (row-major n x n).
+static void sym_eig(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z) {
+  Z.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+  auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        tot += a(i, j) * a(i, j);
+        if (i != j) off += a(i, j) * a(i, j);
+      }
+    if (off <= 1e-32 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a(p, q);
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = a(k, p), akq = a(k, q);
+          a(k, p) = c * akp - s * akq;
+          a(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = a(p, k), aqk = a(q, k);
+          a(p, k) = c * apk - s * aqk;
+          a(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double zkp = Z[(size_t)k * n + p], zkq = Z[(size_t)k * n + q];
+          Z[(size_t)k * n + p] = c * zkp - s * zkq;
+          Z[(size_t)k * n + q] = s * zkp + c * zkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+  std::vector<double> Zs((size_t)n * n);
+  w.resize(n);
+  for (int c = 0; c < n; ++c) {
+    w[c] = a(idx[c], idx[c]);
+    for (int r = 0; r < n; ++r) Zs[(size_t)r * n + c] = Z[(size_t)r * n + idx[c]];
+  }
+  Z.swap(Zs);
+}
+
+// Lowest eigenvalue of the symmetric tridiagonal (alpha, beta) by Sturm
+// bisection.
+static double tridiag_lowest(const std::vector<double>& al, const std::vector<double>& be) {
+  const int n = (int)al.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < n; ++i) {
+    double r = (i > 0 ? std::fabs(be[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(be[i]) : 0.0);
+    lo = std::min(lo, al[i] - r);
+    hi = std::max(hi, al[i] + r);
+  }
+  auto count_below = [&](double x) {
+    int cnt = 0;
+    double d = 1.0;
+    for (int i = 0; i < n; ++i) {
+      const double b2 = i > 0 ? be[i - 1] * be[i - 1] : 0.0;
+      d = al[i] - x - (i > 0 ? b2 / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0) ++cnt;
+    }
+    return cnt;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+}  // namespace lsqed
+
+using namespace lsqed;
+
+struct lsqed_eps {
+  lsqed_ham* h = nullptr;
+  int nev = 1, ncv = 0, mpd = 0, maxit = 0;
+  double tol = 1e-8;
+  uint64_t seed = 1;
+  double* V = nullptr;     // (ncv + 1) columns of D
+  double* part = nullptr;  // kMaxNcv * kRedBlocks partials
+  double* dsm = nullptr;   // small device buffer: coefficients / S matrix
+  std::vector<double> evals;
+  int nconv = 0, its = 0;
+  bool solved = false;
+};
+
+namespace {
+
+struct Ctx {
+  lsqed_eps* e;
+  hipStream_t s;
+  int64_t D;
+};
+
+// h = V[:, 0:m]^T w (host out), device copy left in e->dsm[0:m].
+int multidot(Ctx& c, int m, const double* w, std::vector<double>& h) {
+  const int G = red_blocks(c.D);
+  const int groups = (m + 15) / 16;
+  hipLaunchKernelGGL(k_multidot, dim3(G, groups), dim3(kThreads), 0, c.s, c.e->V, c.D, m, w, c.D, c.e->part);
+  LSQED_HIP(hipGetLastError());
+  h.resize(m);
+  return finish_reduce(c.e->h, c.e->part, G, m, c.e->dsm, h.data(), c.s);
+}
+
+// w -= V[:, 0:m] h ; returns ||w|| (h given on host, uploaded).
+int update(Ctx& c, int m, double* w, const std::vector<double>& h, double& nrm) {
+  const int G = red_blocks(c.D);
+  LSQED_HIP(hipMemcpyAsync(c.e->dsm, h.data(), m * sizeof(double), hipMemcpyHostToDevice, c.s));
+  hipLaunchKernelGGL(k_update, dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, m, c.e->dsm, w, c.D, c.e->part);
+  LSQED_HIP(hipGetLastError());
+  double ss = 0.0;
+  int rc = finish_reduce(c.e->h, c.e->part, G, 1, c.e->dsm + kMaxNcv, &ss, c.s);
+  if (rc) return rc;
+  nrm = std::sqrt(ss);
+  return LSQED_OK;
+}
+
+int norm2(Ctx& c, const double* x, double& nrm) {
+  const int G = red_blocks(c.D);
+  hipLaunchKernelGGL(k_dot, dim3(G), dim3(kThreads), 0, c.s, x, x, c.D, c.e->part);
+  LSQED_HIP(hipGetLastError());
+  double ss = 0.0;
+  int rc = finish_reduce(c.e->h, c.e->part, G, 1, c.e->dsm, &ss, c.s);
+  nrm = std::sqrt(ss);
+  return rc;
+}
+
+int scale(Ctx& c, double* x, double a) {
+  hipLaunchKernelGGL(k_scale, dim3(stream_grid(c.D)), dim3(kThreads), 0, c.s, x, c.D, a);
+  LSQED_HIP(hipGetLastError());
+  return LSQED_OK;
+}
+
+// Classical Gram-Schmidt, twice, of w against V[:, 0:m]; h accumulates both
+// passes, nrm = ||w|| afterwards.
+int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm) {
+  std::vector<double> h1, h2;
+  int rc;
+  if (m == 0) return norm2(c, w, nrm);
+  if ((rc = multidot(c, m, w, h1))) return rc;
+  if ((rc = update(c, m, w, h1, nrm))) return rc;
+  if ((rc = multidot(c, m, w, h2))) return rc;
+  if ((rc = update(c, m, w, h2, nrm))) return rc;
+  h.resize(m);
+  for (int j = 0; j < m; ++j) h[j] = h1[j] + h2[j];
+  return LSQED_OK;
+}
+
+// V[:, 0:k] = V[:, 0:m] S[:, 0:k] (S host row-major m x m, first k columns).
+int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
+  std::vector<double> Sc((size_t)m * k);
+  for (int col = 0; col < k; ++col)
+    for (int j = 0; j < m; ++j) Sc[(size_t)col * m + j] = S[(size_t)j * m + col];
+  LSQED_HIP(hipMemcpyAsync(c.e->dsm, Sc.data(), Sc.size() * sizeof(double), hipMemcpyHostToDevice, c.s));
+  const int g = stream_grid(c.D);
+  if (m <= 16)
+    hipLaunchKernelGGL(k_restart<16>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+  else if (m <= 32)
+    hipLaunchKernelGGL(k_restart<32>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+  else
+    hipLaunchKernelGGL(k_restart<64>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+  LSQED_HIP(hipGetLastError());
+  return LSQED_OK;
+}
+
+// New random start vector for column k, orthonormal to V[:, 0:k].
+int fresh_vector(Ctx& c, int k, uint64_t seed) {
+  double* v = c.e->V + (int64_t)k * c.D;
+  hipLaunchKernelGGL(k_fill_random, dim3(stream_grid(c.D)), dim3(kThreads), 0, c.s, v, c.D, seed);
+  LSQED_HIP(hipGetLastError());
+  std::vector<double> h;
+  double nrm = 0.0;
+  int rc = cgs2(c, k, v, h, nrm);
+  if (rc) return rc;
+  if (!(nrm > 0.0)) return fail(LSQED_ERR_STATE, "start vector vanished after orthogonalisation");
+  return scale(c, v, 1.0 / nrm);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out) {
+  if (!h || !out) return fail(LSQED_ERR_ARG, "null argument");
+  *out = nullptr;
+  lsqed_eps_opts d{};
+  d.nev = 1;
+  if (o) d = *o;
+  if (d.nev <= 0) d.nev = 1;
+  const int64_t D = h->D;
+  if (D <= 0) return fail(LSQED_ERR_ARG, "empty sector");
+  if (d.nev > D) return fail(LSQED_ERR_ARG, "nev larger than the sector dimension");
+  int ncv = d.ncv > 0 ? d.ncv : std::max(2 * d.nev, d.nev + 15);  // SLEPc default
+  ncv = (int)std::min<int64_t>(ncv, D);
+  if (ncv > kMaxNcv) return fail(LSQED_ERR_ARG, "ncv > 64 is not supported");
+  if (ncv < d.nev) return fail(LSQED_ERR_ARG, "ncv must be >= nev");
+  lsqed_eps* e = new lsqed_eps;
+  e->h = h;
+  e->nev = d.nev;
+  e->ncv = ncv;
+  e->mpd = d.mpd > 0 ? d.mpd : ncv;
+  e->tol = d.tol > 0 ? d.tol : 1e-8;
+  e->maxit = d.maxit > 0 ? d.maxit : (int)std::max<int64_t>(100, 2 * D / ncv);
+  e->seed = d.seed ? d.seed : 1;
+  const size_t bytes = (size_t)(ncv + 1) * D * sizeof(double);
+  if (hipMalloc((void**)&e->V, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete e;
+    return fail(LSQED_ERR_OOM, "cannot allocate the Krylov basis ((ncv+1) x D doubles)");
+  }
+  if (hipMalloc((void**)&e->part, (size_t)kMaxNcv * kRedBlocks * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&e->dsm, (size_t)(kMaxNcv * kMaxNcv + 2 * kMaxNcv) * sizeof(double)) != hipSuccess) {
+    lsqed_eps_destroy(e);
+    return fail(LSQED_ERR_OOM, "cannot allocate solver scratch");
+  }
+  *out = e;
+  return LSQED_OK;
+}
+
+int lsqed_eps_destroy(lsqed_eps* e) {
+  if (!e) return LSQED_OK;
+  (void)hipFree(e->V);
+  (void)hipFree(e->part);
+  (void)hipFree(e->dsm);
+  delete e;
+  return LSQED_OK;
+}
+
+int lsqed_eps_iterations(const lsqed_eps* e) { return e ? e->its : 0; }
+
+int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
+  if (!e) return fail(LSQED_ERR_ARG, "null solver");
+  Ctx c{e, (hipStream_t)stream, e->h->D};
+  const int m = e->ncv;
+  const int64_t D = c.D;
+  int rc;
+  auto col = [&](int j) { return e->V + (int64_t)j * D; };
+
+  // start vector (SLEPc draws a random one; ours is a seeded hash)
+  hipLaunchKernelGGL(k_fill_random, dim3(stream_grid(D)), dim3(kThreads), 0, c.s, col(0), D, e->seed);
+  LSQED_HIP(hipGetLastError());
+  double nrm;
+  if ((rc = norm2(c, col(0), nrm))) return rc;
+  if ((rc = scale(c, col(0), 1.0 / nrm))) return rc;
+
+  std::vector<double> T((size_t)m * m, 0.0), arrow;
+  std::vector<double> beta(m + 1, 0.0);
+  int k = 0;
+  double tnorm = 0.0;
+  e->its = 0;
+  e->nconv = 0;
+  e->evals.clear();
+  uint64_t reseed = e->seed;
+  while (true) {
+    int meff = m;
+    bool breakdown = false;
+    for (int j = k; j < m; ++j) {
+      if ((rc = launch_hv(e->h, col(j), col(j + 1), nullptr, 0, c.s))) return rc;
+      std::vector<double> hcoef;
+      double b;
+      if ((rc = cgs2(c, j + 1, col(j + 1), hcoef, b))) return rc;
+      T[(size_t)j * m + j] = hcoef[j];
+      if (j == k) {
+        for (int i = 0; i < k; ++i) T[(size_t)i * m + k] = T[(size_t)k * m + i] = arrow[i];
+      } else {
+        T[(size_t)j * m + j - 1] = T[(size_t)(j - 1) * m + j] = beta[j - 1];
+      }
+      tnorm = std::max(tnorm, std::fabs(hcoef[j]) + b);
+      beta[j] = b;
+      if (b <= 1e-13 * tnorm || b == 0.0) {
+        meff = j + 1;
+        breakdown = true;
+        break;
+      }
+      if ((rc = scale(c, col(j + 1), 1.0 / b))) return rc;
+    }
+    ++e->its;
+    // projected problem
+    std::vector<double> A((size_t)meff * meff), w, S;
+    for (int i = 0; i < meff; ++i)
+      for (int j = 0; j < meff; ++j) A[(size_t)i * meff + j] = T[(size_t)i * m + j];
+    sym_eig(meff, A, w, S);
+    std::vector<double> res(meff);
+    for (int i = 0; i < meff; ++i)
+      res[i] = breakdown ? 0.0 : std::fabs(beta[meff - 1] * S[(size_t)(meff - 1) * meff + i]);
+    int nconv = 0;
+    while (nconv < meff && res[nconv] <= e->tol * std::max(std::fabs(w[nconv]), 1e-300)) ++nconv;
+    const bool enough = nconv >= e->nev;
+    const bool full = breakdown && meff >= m;
+    if (enough || e->its >= e->maxit || full) {
+      int kfin = enough ? nconv : std::max(nconv, std::min(e->nev, meff));
+      kfin = std::min(kfin, meff);
+      if ((rc = restart_transform(c, meff, kfin, S))) return rc;
+      e->nconv = nconv;
+      e->evals.assign(w.begin(), w.begin() + kfin);
+      break;
+    }
+    if (breakdown) {
+      // Invariant subspace: keep every Ritz vector, continue from a fresh
+      // random direction orthogonal to them (SLEPc restarts the same way).
+      if ((rc = restart_transform(c, meff, meff, S))) return rc;
+      k = meff;
+      arrow.assign(k, 0.0);
+      std::fill(T.begin(), T.end(), 0.0);
+      for (int i = 0; i < k; ++i) T[(size_t)i * m + i] = w[i];
+      if ((rc = fresh_vector(c, k, ++reseed))) return rc;
+      continue;
+    }
+    int keep = nconv + std::max(1, (meff - nconv) / 2);
+    keep = std::min(keep, meff - 1);
+    if ((rc = restart_transform(c, meff, keep, S))) return rc;
+    LSQED_HIP(hipMemcpyAsync(col(keep), col(meff), D * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+    arrow.resize(keep);
+    for (int i = 0; i < keep; ++i) arrow[i] = beta[meff - 1] * S[(size_t)(meff - 1) * meff + i];
+    std::fill(T.begin(), T.end(), 0.0);
+    for (int i = 0; i < keep; ++i) T[(size_t)i * m + i] = w[i];
+    k = keep;
+  }
+  LSQED_HIP(hipStreamSynchronize(c.s));
+  e->solved = true;
+  if (nconv_out) *nconv_out = e->nconv;
+  return LSQED_OK;
+}
+
+int lsqed_eps_solution(lsqed_eps* e, int i, double* lambda, double* x, double* error, lsqed_stream_t stream) {
+  if (!e) return fail(LSQED_ERR_ARG, "null solver");
+  if (!e->solved) return fail(LSQED_ERR_STATE, "lsqed_eps_solve has not run");
+  if (i < 0 || i >= (int)e->evals.size()) return fail(LSQED_ERR_ARG, "eigenpair index out of range");
+  Ctx c{e, (hipStream_t)stream, e->h->D};
+  const int64_t D = c.D;
+  const double lam = e->evals[i];
+  if (lambda) *lambda = lam;
+  double* vi = e->V + (int64_t)i * D;
+  if (x) LSQED_HIP(hipMemcpyAsync(x, vi, D * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  if (error) {
+    // EPSComputeError(EPS_ERROR_RELATIVE): ||H x - lambda x|| / |lambda|
+    double* y = e->V + (int64_t)e->ncv * D;
+    int rc = launch_hv(e->h, vi, y, nullptr, 0, c.s);
+    if (rc) return rc;
+    const int G = red_blocks(D);
+    hipLaunchKernelGGL(k_resid, dim3(G), dim3(kThreads), 0, c.s, y, vi, lam, D, e->part);
+    LSQED_HIP(hipGetLastError());
+    double ss = 0.0;
+    if ((rc = finish_reduce(e->h, e->part, G, 1, e->dsm, &ss, c.s))) return rc;
+    *error = std::sqrt(ss) / std::max(std::fabs(lam), 1e-300);
+  }
+  LSQED_HIP(hipStreamSynchronize(c.s));
+  return LSQED_OK;
+}
+
+int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta, double* y,
+                       double* alpha, double* beta_next, lsqed_stream_t stream) {
+  if (!h || !q || !p || !y) return fail(LSQED_ERR_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t D = h->D;
+  const int64_t ntiles = (D + 255) / 256;
+  const int G = hv_grid(h, ntiles);
+  const int R = red_blocks(D);
+  int rc = ensure_scratch(h, (size_t)G + R + 8);
+  if (rc) return rc;
+  double* dpart = h->d_scratch;
+  double* upart = h->d_scratch + G;
+  double* outv = h->d_scratch + G + R;
+  if ((rc = launch_hv(h, q, y, dpart, G, s))) return rc;
+  double qy = 0.0;
+  if ((rc = finish_reduce(h, dpart, G, 1, outv, &qy, s))) return rc;
+  const double a = sq * sq * qy;
+  hipLaunchKernelGGL(k_lanczos_update, dim3(R), dim3(kThreads), 0, s, y, q, p, D, sq, -a * sq, -beta * sp, upart);
+  LSQED_HIP(hipGetLastError());
+  double ss = 0.0;
+  if ((rc = finish_reduce(h, upart, R, 1, outv + 1, &ss, s))) return rc;
+  if (alpha) *alpha = a;
+  if (beta_next) *beta_next = std::sqrt(ss);
+  return LSQED_OK;
+}
+
+int lsqed_lanczos_ground(lsqed_ham* h, int maxit, double tol, uint64_t seed, double* work, double* e0, int* steps,
+                         lsqed_stream_t stream) {
+  if (!h || !work || !e0) return fail(LSQED_ERR_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t D = h->D;
+  double* q = work;
+  double* p = work + D;
+  double* y = work + 2 * D;
+  hipLaunchKernelGGL(k_fill_random, dim3(stream_grid(D)), dim3(kThreads), 0, s, q, D, seed ? seed : 1);
+  LSQED_HIP(hipGetLastError());
+  LSQED_HIP(hipMemsetAsync(p, 0, D * sizeof(double), s));
+  // ||q||
+  const int R = red_blocks(D);
+  int rc = ensure_scratch(h, (size_t)std::max(R, 2048) + 8);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dot, dim3(R), dim3(kThreads), 0, s, q, q, D, h->d_scratch);
+  LSQED_HIP(hipGetLastError());
+  double ss = 0.0;
+  if ((rc = finish_reduce(h, h->d_scratch, R, 1, h->d_scratch + 2048, &ss, s))) return rc;
+  double sq = 1.0 / std::sqrt(ss), sp = 0.0, beta = 0.0;
+  std::vector<double> al, be;
+  std::vector<double> hist;
+  if (maxit <= 0) maxit = 1000;
+  if (tol <= 0) tol = 1e-12;
+  int it = 0;
+  double ev = 0.0;
+  for (it = 0; it < maxit; ++it) {
+    double a, bn;
+    if ((rc = lsqed_lanczos_step(h, q, sq, p, sp, beta, y, &a, &bn, stream))) return rc;
+    al.push_back(a);
+    ev = tridiag_lowest(al, be);
+    hist.push_back(ev);
+    const int n = (int)hist.size();
+    if (n > 5 && std::fabs(hist[n - 1] - hist[n - 6]) <= tol * std::max(1.0, std::fabs(ev))) {
+      ++it;
+      break;
+    }
+    if (!(bn > 1e-14 * std::fabs(a)) || (int64_t)al.size() >= D) {
+      ++it;
+      break;
+    }
+    be.push_back(bn);
+    // new q = p / bn (scale factor), new p = old q
+    std::swap(q, p);
+    sp = sq;
+    sq = 1.0 / bn;
+    beta = bn;
+  }
+  *e0 = ev;
+  if (steps) *steps = it;
+  return LSQED_OK;
+}
+
+}  // extern "C"

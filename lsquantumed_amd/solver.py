@@ -1,0 +1,98 @@
+"""Solver namespace of LSQuantumED (include/solver.hpp, src/solver.cpp) over
+the device-resident Krylov-Schur of liblsqed.so.
+
+    eps = Solver.SolverInit(h, nev, ncv, mpd)      # src/solver.cpp:7-24
+    nconv = Solver.solve(env, eps)                 # :38-52
+    lam, err = Solver.solution(eps, i, vec)        # :58-68 (vec: CUDA tensor or None)
+    Solver.SolverClean(eps)                        # :28-32
+
+Extra knobs that the reference takes from the PETSc options database
+(``-eps_tol``, ``-eps_max_it``; EPSSetFromOptions, src/solver.cpp:21) are
+keyword arguments here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+
+PETSC_DEFAULT = -2
+PETSC_DECIDE = -1
+
+
+class EPS:
+    """Handle of one eigensolve (the reference's `EPS` object)."""
+
+    def __init__(self, h, nev, ncv=PETSC_DEFAULT, mpd=PETSC_DEFAULT, tol=None, maxit=None, seed=1):
+        self.h = h
+        o = _lib.EpsOpts(int(nev), int(ncv) if ncv and ncv > 0 else 0, int(mpd) if mpd and mpd > 0 else 0,
+                         float(tol) if tol else 0.0, int(maxit) if maxit else 0, int(seed))
+        e = C.c_void_p()
+        check(lib().lsqed_eps_create(h.hamilt, C.byref(o), C.byref(e)))
+        self.handle = e
+        self.nconv = 0
+
+    def destroy(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().lsqed_eps_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def iterations(self) -> int:
+        return lib().lsqed_eps_iterations(self.handle)
+
+
+class Solver:
+    @staticmethod
+    def SolverInit(h, nev, ncv=PETSC_DEFAULT, mpd=PETSC_DEFAULT, tol=None, maxit=None, seed=1) -> EPS:
+        return EPS(h, nev, ncv, mpd, tol, maxit, seed)
+
+    @staticmethod
+    def solve(env, eps: EPS, stream=None) -> int:
+        n = C.c_int()
+        check(lib().lsqed_eps_solve(eps.handle, C.byref(n), stream_ptr(stream)))
+        eps.nconv = n.value
+        return n.value
+
+    @staticmethod
+    def solution(eps: EPS, index: int, vec=None, stream=None):
+        lam = C.c_double()
+        err = C.c_double()
+        vp = ptr(vec) if vec is not None else None
+        check(lib().lsqed_eps_solution(eps.handle, int(index), C.byref(lam), vp, C.byref(err), stream_ptr(stream)))
+        return lam.value, err.value
+
+    @staticmethod
+    def SolverClean(eps: EPS) -> int:
+        eps.destroy()
+        return 0
+
+
+def lanczos_ground(h, maxit=2000, tol=1e-13, seed=1, work=None, stream=None):
+    """Ground energy from the three-vector Lanczos (no eigenvector).  `work`:
+    optional float64 CUDA tensor of 3*D elements."""
+    import torch
+
+    D = h.hamilt_size()
+    if work is None:
+        work = torch.empty(3 * D, dtype=torch.float64, device="cuda")
+    e0 = C.c_double()
+    steps = C.c_int()
+    check(lib().lsqed_lanczos_ground(h.hamilt, int(maxit), float(tol), int(seed), ptr(work), C.byref(e0),
+                                     C.byref(steps), stream_ptr(stream)))
+    return e0.value, steps.value
+
+
+def lanczos_step(h, q, sq, p, sp, beta, y, stream=None):
+    """One three-term Lanczos step (see include/lsqed.h); returns (alpha, beta')."""
+    a = C.c_double()
+    b = C.c_double()
+    check(lib().lsqed_lanczos_step(h.hamilt, ptr(q), float(sq), ptr(p), float(sp), float(beta), ptr(y), C.byref(a),
+                                   C.byref(b), stream_ptr(stream)))
+    return a.value, b.value

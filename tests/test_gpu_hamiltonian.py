@@ -1,0 +1,220 @@
+"""GPU parity of the matrix-free Hamiltonian against the oracle.
+
+Bit-exact: basis states (unrank), ranks, column lists in emission order,
+the stored diagonal.  FP64 H.v: within 1e-13 relative (summation order
+differs from a CSR row sum).  Every case prints its name first so a failure
+is attributable."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LT = {"chain": 0, "square": 1, "honeycomb": 2}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as T
+    assert T.cuda.is_available(), "GPU tests need a GPU"
+    return T
+
+
+def make(lat, N, mag, J1=1.0, D1=1.0, J2=0.0, D2=0.0, lx=0, ly=0, builds=("build_diag", "build_off_diag"), hi=None):
+    import lsquantumed_amd as Q
+    env = Q.Environment(N)
+    b = Q.Basis(env, mag)
+    L = {"chain": lambda: Q.chain1D(env), "square": lambda: Q.square2D(env, lx, ly),
+         "honeycomb": lambda: Q.honeycomb2D(env, lx, ly)}[lat]()
+    h = Q.Hamiltonian(env, b, L, J1, D1, J2, D2)
+    if "build_diag" in builds:
+        h.build_diag(env)
+    if "build_off_diag" in builds:
+        h.build_off_diag(env)
+    if "build_disorder" in builds:
+        h.build_disorder(env, hi)
+    return h
+
+
+def oracle_model(O, lat, N, mag, J1=1.0, D1=1.0, J2=0.0, D2=0.0, lx=0, ly=0):
+    return O.Model(LT[lat], N, mag + N // 2, J1, D1, J2, D2, lx=lx, ly=ly)
+
+
+def flags_of(O, builds):
+    f = 0
+    f |= O.Model.BUILD_DIAG if "build_diag" in builds else 0
+    f |= O.Model.BUILD_OFFDIAG if "build_off_diag" in builds else 0
+    f |= O.Model.BUILD_DISORDER if "build_disorder" in builds else 0
+    return f
+
+
+@pytest.mark.parametrize("N,mag", [(6, 0), (6, -2), (8, 1), (12, 0), (16, 0), (20, 3), (24, 0), (34, -13), (36, -15)])
+def test_unrank_rank_bitexact(O, torch, N, mag):
+    say("unrank/rank", N, mag)
+    h = make("chain", N, mag)
+    D = h.hamilt_size()
+    nup = mag + N // 2
+    assert D == math.comb(N, nup)
+    n = min(D, 200000)
+    st = h.unrank(0, n).cpu().numpy().astype(np.uint64)
+    assert np.array_equal(st, O.int_basis(N, nup, 0, n))
+    r = h.rank(torch.from_numpy(st.astype(np.int64)).cuda()).cpu().numpy()
+    assert np.array_equal(r, np.arange(n))
+    if D > n:  # the tail of the sector
+        st2 = h.unrank(D - 1000, 1000).cpu().numpy().astype(np.uint64)
+        assert np.array_equal(st2, O.int_basis(N, nup, D - 1000, 1000))
+
+
+@pytest.mark.parametrize("N,lx,ly", [(32, 4, 8), (36, 6, 6)])
+def test_unrank_rank_large_sector_sample(O, torch, N, lx, ly):
+    say("unrank/rank sample", N)
+    h = make("square", N, 0, lx=lx, ly=ly)
+    D = h.hamilt_size()
+    rng = np.random.default_rng(N)
+    for row0 in [0, D // 7, D // 2, D - 4096] + [int(v) for v in rng.integers(0, D - 4096, 4)]:
+        st = h.unrank(row0, 4096).cpu().numpy()
+        ref = O.int_basis(N, N // 2, row0, 4096).astype(np.int64)
+        assert np.array_equal(st, ref), row0
+        assert np.array_equal(h.rank(torch.from_numpy(st).cuda()).cpu().numpy(), np.arange(row0, row0 + 4096))
+
+
+def test_export_cols_reference_literals(O, torch, literals):
+    """Every get_coup_elems literal of the reference's tests (emission order,
+    duplicates included), through the device column export."""
+    say("export literals", len(literals["get_coup_elems"]))
+    cache = {}
+    for c in literals["get_coup_elems"]:
+        key = (c["lattice"], c["N"], c["mag"], c["lx"], c["ly"])
+        if key not in cache:
+            cache[key] = make(c["lattice"], c["N"], c["mag"], lx=c["lx"], ly=c["ly"])
+        h = cache[key]
+        kind = 0 if c["kind"] == "nn" else 1
+        cols, ncol = h.export_cols(kind, c["global_index"], 1)
+        n = int(ncol[0])
+        assert n == c["ncol"], c["src"]
+        assert cols[0, :n].cpu().tolist() == c["cols"], c["src"]
+
+
+CASES = [
+    ("chain", 6, 0, 1, 1, 0, 0, 0, 0), ("chain", 6, -1, 1, 1, 0, 0, 0, 0), ("chain", 8, 0, 1, 0.3456, 0.7, 0.2, 0, 0),
+    ("chain", 16, 0, 1, 1, 0, 0, 0, 0), ("chain", 18, 0, 1, 1, 0.2, 0.2, 0, 0),
+    ("square", 8, 2, 1, 1, 1, 1, 2, 4), ("square", 12, 5, 1, 1, 1, 1, 3, 4), ("square", 16, 0, 1, -0.43, -1, 1, 4, 4),
+    ("square", 20, 0, 1, 1, 0.5, 0.5, 4, 5), ("honeycomb", 12, 5, 1, 1, 1, 1, 6, 2),
+    ("honeycomb", 8, 2, 1, 1, 0, 0, 2, 4), ("square", 6, 0, 1, 1, 1, 1, 6, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}m{c[2]}" for c in CASES])
+def test_export_cols_bitexact(O, torch, case):
+    lat, N, mag, J1, D1, J2, D2, lx, ly = case
+    say("export", case)
+    h = make(lat, N, mag, J1, D1, J2, D2, lx, ly)
+    D = h.hamilt_size()
+    nup = mag + N // 2
+    states = O.int_basis(N, nup)
+    for kind in (0, 1):
+        bi, bj = O.bonds(LT[lat], N, lx, ly, kind)
+        cols, ncol = h.export_cols(kind)
+        cols, ncol = cols.cpu().numpy(), ncol.cpu().numpy()
+        for r in range(D):
+            ref = O.coup_elems(int(states[r]), N, bi, bj)
+            assert ncol[r] == len(ref) and list(cols[r, :ncol[r]]) == ref, (kind, r)
+
+
+def test_diag_fixtures_bitexact(O, torch, literals, refdata):
+    """build_diag / build_disorder dumps of the reference, through lsqed_diag."""
+    for c in literals["compare"]:
+        if c["view"] != "vec" and "build_disorder" not in c["builds"]:
+            continue
+        say("diag fixture", c["ref"])
+        hi = np.arange(c["N"]) / c["N"] if c["hi"] else None
+        h = make(c["lattice"], c["N"], c["mag"], c["J1"], c["D1"], c["J2"], c["D2"], c["lx"], c["ly"],
+                 builds=c["builds"], hi=hi)
+        d = h.diagonal().cpu().numpy()
+        m = oracle_model(O, c["lattice"], c["N"], c["mag"], c["J1"], c["D1"], c["J2"], c["D2"], c["lx"], c["ly"])
+        dref, _ = m.diag(hi, flags_of(O, c["builds"]))
+        assert np.array_equal(d, dref), c["ref"]
+        if c["view"] == "vec":
+            assert [O.petsc_g(v) for v in d] == O.parse_petsc_vec(refdata(c["ref"]))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}m{c[2]}" for c in CASES])
+@pytest.mark.parametrize("fields", [False, True])
+def test_hv_matches_oracle_csr(O, torch, case, fields):
+    lat, N, mag, J1, D1, J2, D2, lx, ly = case
+    say("hv", case, fields)
+    hi = np.random.default_rng(N).uniform(-2, 2, N) if fields else None
+    builds = ("build_diag", "build_off_diag", "build_disorder") if fields else ("build_diag", "build_off_diag")
+    h = make(lat, N, mag, J1, D1, J2, D2, lx, ly, builds=builds, hi=hi)
+    m = oracle_model(O, lat, N, mag, J1, D1, J2, D2, lx, ly)
+    A = m.csr(hi, flags_of(O, builds))
+    D = h.hamilt_size()
+    x = np.random.default_rng(D).uniform(-1, 1, D)
+    y = h.matvec(torch.from_numpy(x).cuda()).cpu().numpy()
+    yref = A @ x
+    scale = np.abs(A).max() * np.abs(x).max()
+    assert np.max(np.abs(y - yref)) <= 1e-13 * scale
+    # diagonal entries bit-exact as well
+    d = h.diagonal().cpu().numpy()
+    assert np.array_equal(d, A.diagonal())
+
+
+def test_build_flags(O, torch):
+    """The operator holds only what the build calls added (H starts empty)."""
+    say("build flags")
+    import lsquantumed_amd as Q
+    env = Q.Environment(8)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.chain1D(env), 1.0, 1.0, 0.0, 0.0)
+    D = h.hamilt_size()
+    x = torch.rand(D, dtype=torch.float64, device="cuda")
+    assert torch.count_nonzero(h.matvec(x)) == 0
+    h.build_off_diag(env)
+    m = O.Model(0, 8, 4, 1.0, 1.0, 0.0, 0.0)
+    assert np.allclose(h.matvec(x).cpu().numpy(), m.csr(None, 2) @ x.cpu().numpy(), atol=1e-14)
+
+
+@pytest.mark.parametrize("lat,N,lx,ly,J2", [("square", 24, 4, 6, 0.5), ("chain", 22, 0, 0, 0.2)])
+def test_hv_config_sizes_matfree(O, torch, lat, N, lx, ly, J2):
+    """Full-size H.v of the 4x6 J1-J2 lattice and the N=22 chain against the
+    oracle's matrix-free product (same row semantics, exact binomials)."""
+    say("hv config", lat, N)
+    h = make(lat, N, 0, 1.0, 1.0, J2, J2, lx, ly)
+    m = oracle_model(O, lat, N, 0, 1.0, 1.0, J2, J2, lx, ly)
+    D = h.hamilt_size()
+    x = np.random.default_rng(5).uniform(-1, 1, D)
+    y = h.matvec(torch.from_numpy(x).cuda()).cpu().numpy()
+    yref = m.hv_matfree(x)
+    assert np.max(np.abs(y - yref)) <= 1e-13 * np.abs(yref).max()
+
+
+def test_hv_4x8_properties_and_rows(O, torch):
+    """4x8 J1-J2 (D = 601,080,390): rows sampled across the sector against
+    the oracle's matrix-free rows, symmetry <x, H y> = <H x, y>, linearity."""
+    say("hv 4x8")
+    h = make("square", 32, 0, 1.0, 1.0, 0.5, 0.5, 4, 8)
+    m = oracle_model(O, "square", 32, 0, 1.0, 1.0, 0.5, 0.5, 4, 8)
+    D = h.hamilt_size()
+    assert D == 601080390
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(D, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    y = h.matvec(x)
+    xh = x.cpu().numpy()
+    yh = y.cpu().numpy()
+    for row0 in (0, 123456789, D // 2, D - 5000):
+        yref = m.hv_matfree(xh, row0=row0, nrows=5000)
+        assert np.max(np.abs(yh[row0:row0 + 5000] - yref)) <= 1e-12 * np.abs(yref).max(), row0
+    del xh, yh
+    z = torch.rand(D, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    hz = h.matvec(z)
+    a = torch.dot(z, y).item()
+    b = torch.dot(hz, x).item()
+    assert abs(a - b) <= 1e-10 * (abs(a) + abs(b))
+    w = h.matvec(2.0 * x - 3.0 * z)
+    assert torch.max(torch.abs(w - (2.0 * y - 3.0 * hz))).item() <= 1e-11 * torch.max(torch.abs(w)).item()

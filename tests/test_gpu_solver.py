@@ -1,0 +1,171 @@
+"""GPU parity of the eigensolver and observables.
+
+Eigenvalues against the reference's literals (test_solver1-5, test_dsf1,
+correlation tests) and against the oracle's dense/sparse spectra;
+correlators against the reference's literals; DSF against dsf_d0_{1,2}.ref
+(energies and phase-invariant multiplet weights)."""
+import sys
+
+import numpy as np
+import pytest
+
+from test_gpu_hamiltonian import LT, flags_of, make, oracle_model
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as T
+    assert T.cuda.is_available()
+    return T
+
+
+def solve(h, nev, ncv=-2, mpd=-2, tol=1e-12, vectors=False):
+    import torch
+    from lsquantumed_amd import Solver
+    eps = Solver.SolverInit(h, nev, ncv, mpd, tol=tol)
+    nconv = Solver.solve(None, eps)
+    vals, errs, vecs = [], [], []
+    for i in range(min(nconv, max(nev, nconv))):
+        v = torch.empty(h.hamilt_size(), dtype=torch.float64, device="cuda") if vectors else None
+        lam, err = Solver.solution(eps, i, v)
+        vals.append(lam)
+        errs.append(err)
+        vecs.append(v)
+    Solver.SolverClean(eps)
+    return nconv, np.array(vals), np.array(errs), vecs
+
+
+def test_solver_reference_literals(O, torch, literals):
+    for c in literals["solver"]:
+        say("solver", c["test"])
+        h = make(c["lattice"], c["N"], c["mag"], c["J1"], c["D1"], c["J2"], c["D2"], c["lx"], c["ly"],
+                 builds=c["builds"])
+        nconv, vals, errs, _ = solve(h, c["nev"], c["ncv"] or -2, c["mpd"] or -2, tol=1e-12)
+        assert nconv >= min(len(c["eigs"]), c["nev"]), (c["test"], nconv)
+        for k, e in enumerate(c["eigs"][:nconv]):
+            assert abs(vals[k] - e) <= max(c["tol"], 1e-9 * abs(e)), (c["test"], k, vals[k], e)
+            assert errs[k] <= 1e-8, (c["test"], k, errs[k])
+
+
+def test_chain16_ground_state_and_szsz(O, torch):
+    """Chain N=16 J1=D1=1 (D=12,870): E0 and <S^z_0 S^z_r> against the oracle."""
+    say("chain16")
+    h = make("chain", 16, 0)
+    nconv, vals, errs, vecs = solve(h, 1, tol=1e-10, vectors=True)
+    m = oracle_model(O, "chain", 16, 0)
+    w, v = O.eigs(m.csr(None, 3), 1)
+    assert abs(vals[0] - w[0]) <= 1e-9 * abs(w[0])
+    assert abs(vals[0] - (-7.142296360616785)) < 1e-9
+    from lsquantumed_amd import correlations
+    c = correlations(h, vecs[0])
+    for r in range(16):
+        assert abs(c["SzSz"][0, r] - O.szsz(16, 8, v[:, 0], 0, r)) <= 1e-8
+
+
+def test_correlation_literals(O, torch, literals):
+    from lsquantumed_amd import correlations
+    for c in literals["correlation"]:
+        say("correlation", c["src"])
+        h = make(c["lattice"], c["N"], c["mag"], c["J1"], c["D1"], c["J2"], c["D2"], c["lx"], c["ly"])
+        nconv, vals, errs, vecs = solve(h, 1, tol=1e-12, vectors=True)
+        assert abs(vals[0] - c["E0"]) < 1e-11
+        cc = correlations(h, vecs[0])
+        mat = cc["SzSz"] if c["op"] == "SzSz" else cc["SpSm"]
+        for k, ref in enumerate(c["values"]):
+            got = mat[c["i"], k + c["j_offset"]]
+            assert abs(got - ref) <= c["tol"], (c["src"], k, got, ref)
+
+
+def test_all_pair_correlators_vs_oracle(O, torch):
+    """Every (i, j) of SzSz, SpSm, SmSp, SiSj and <S^z_i> with random fields
+    (so that <S^z_i> != 0), 3x4 lattice with J2, mag 1."""
+    say("all pairs")
+    hi = np.random.default_rng(1).uniform(-1, 1, 12)
+    h = make("square", 12, 1, 1.0, 1.0, 0.4, 0.4, 3, 4, builds=("build_diag", "build_off_diag", "build_disorder"),
+             hi=hi)
+    nconv, vals, errs, vecs = solve(h, 1, tol=1e-12, vectors=True)
+    from lsquantumed_amd import correlations
+    c = correlations(h, vecs[0])
+    psi = vecs[0].cpu().numpy()
+    nup = 7
+    for i in range(12):
+        for j in range(12):
+            assert abs(c["SzSz"][i, j] - O.szsz(12, nup, psi, i, j)) < 1e-12
+            assert abs(c["SpSm"][i, j] - O.spsm(12, nup, psi, i, j)) < 1e-12
+            assert abs(c["SmSp"][i, j] - O.smsp(12, nup, psi, i, j)) < 1e-12
+            assert abs(c["SiSj"][i, j] - O.sisj(12, nup, psi, i, j)) < 1e-12
+    sz_ref = [0.5 * np.sum(psi ** 2 * np.where((O.int_basis(12, nup) >> np.uint64(i)) & np.uint64(1), 1, -1))
+              for i in range(12)]
+    assert np.max(np.abs(c["Sz"] - np.array(sz_ref))) < 1e-12
+
+
+def test_magnetisation_vs_oracle(O, torch):
+    say("m2")
+    import lsquantumed_amd as Q
+    env = Q.Environment(16)
+    lat = Q.square2D(env, 4, 4)
+    h = make("square", 16, 0, 1.0, 1.0, 0.5, 0.5, 4, 4)
+    nconv, vals, errs, vecs = solve(h, 1, tol=1e-12, vectors=True)
+    psi = vecs[0].cpu().numpy()
+    maf = Q.Phys.SquaredSubLattMagAF(env, h, Q.AF(lat), vecs[0])
+    mst = Q.Phys.SquaredSubLattMagSTR(env, h, Q.Striped(lat), vecs[0])
+    assert abs(maf - O.m2_af(16, 8, psi, O.af_sublattices(1, 16, 4, 4))) < 1e-10
+    assert abs(mst - O.m2_str(16, 8, psi, O.striped_sublattices(4, 4))) < 1e-10
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_dsf_reference_files(O, torch, literals, refdata, tmp_path, which):
+    """DynStructFactor with Sqz (same sector) and Sqp (Sz 0 -> 1), chain N=6:
+    energies match dsf_d0_{1,2}.ref, multiplet-summed |sum_i e^{iqr} a_i|^2/N
+    match for every q, and the written file has the reference's layout."""
+    from test_oracle import _parse_dsf
+    import lsquantumed_amd as Q
+    c = literals["dsf"][which]
+    say("dsf", c["ref"])
+    env = Q.Environment(6)
+    lat = Q.chain1D(env)
+    h0 = make("chain", 6, c["mag0"])
+    h1 = h0 if c["mag1"] == c["mag0"] else make("chain", 6, c["mag1"])
+    nconv, vals, errs, vecs = solve(h0, 1, tol=1e-12, vectors=True)
+    eigs, amp, _ = Q.Phys.DynStructFactor(env, dict(h0=h0, h1=h1, nev=10, ncv=-2, path=str(tmp_path), tol=1e-12),
+                                          vals[0], vecs[0], 0, op=c["op"])
+    gs, blocks = _parse_dsf(refdata(c["ref"]))
+    assert abs(vals[0] - gs) < 1e-12
+    ref_e = np.array([e for e, _ in blocks[0]])
+    assert np.max(np.abs((eigs - vals[0]) - ref_e)) < 1e-10
+    ref_a = np.array([[blocks[i][n][1] for i in range(6)] for n in range(10)])
+    coords = np.array([[i, 0] for i in range(6)], float)
+    for grp in O.multiplets(eigs):
+        for qx in range(6):
+            q = (2 * np.pi * qx / 6, 0.0)
+            W = sum(O.dsf_weight(amp[n], q, coords) for n in grp)
+            Wref = sum(O.dsf_weight(ref_a[n], q, coords) for n in grp)
+            assert abs(W - Wref) < 1e-9, (grp, qx, W, Wref)
+    gs2, blocks2 = _parse_dsf(open(tmp_path / "dsf_d0.dat").read())
+    assert sorted(blocks2) == sorted(blocks) and all(len(blocks2[i]) == 10 for i in blocks2)
+
+
+def test_lanczos_ground_matches_krylov_schur(O, torch):
+    say("lanczos3")
+    from lsquantumed_amd import lanczos_ground
+    h = make("square", 20, 0, 1.0, 1.0, 0.5, 0.5, 4, 5)
+    e0, steps = lanczos_ground(h, maxit=500, tol=1e-14)
+    nconv, vals, errs, _ = solve(h, 1, tol=1e-12)
+    assert abs(e0 - vals[0]) <= 1e-10 * abs(vals[0]), (e0, vals[0], steps)
+
+
+def test_solver_errors(torch):
+    import lsquantumed_amd as Q
+    h = make("chain", 6, 0)
+    with pytest.raises(Q.LsqedError):
+        Q.Solver.SolverInit(h, 30)  # nev > D
+    eps = Q.Solver.SolverInit(h, 1)
+    with pytest.raises(Q.LsqedError):
+        Q.Solver.solution(eps, 0)  # before solve
