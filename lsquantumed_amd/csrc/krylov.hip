@@ -356,17 +356,30 @@ int scale(Ctx& c, double* x, double a) {
 }
 
 // Classical Gram-Schmidt, twice, of w against V[:, 0:m]; h accumulates both
-// passes, nrm = ||w|| afterwards.
-int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm) {
+// passes.  nrm1 / nrm2 are ||w|| after the first / second pass; the second
+// pass also decides linear dependence the way SLEPc's refined CGS does
+// (BV_ORTHOG_REFINE_IFNEEDED): w is dependent on V when the refinement pass
+// still removes more than 1 - 1/sqrt(2) of its norm.  A residual made of
+// rounding noise is mostly outside span(V), survives the test and becomes the
+// next basis vector -- which is how degenerate copies of an eigenvalue enter
+// the Krylov space.
+int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* lindep = nullptr) {
   std::vector<double> h1, h2;
   int rc;
-  if (m == 0) return norm2(c, w, nrm);
+  if (lindep) *lindep = false;
+  if (m == 0) {
+    rc = norm2(c, w, nrm);
+    if (lindep) *lindep = !(nrm > 0.0);
+    return rc;
+  }
+  double nrm1 = 0.0;
   if ((rc = multidot(c, m, w, h1))) return rc;
-  if ((rc = update(c, m, w, h1, nrm))) return rc;
+  if ((rc = update(c, m, w, h1, nrm1))) return rc;
   if ((rc = multidot(c, m, w, h2))) return rc;
   if ((rc = update(c, m, w, h2, nrm))) return rc;
   h.resize(m);
   for (int j = 0; j < m; ++j) h[j] = h1[j] + h2[j];
+  if (lindep) *lindep = !(nrm > 0.0 && nrm >= 0.7071067811865476 * nrm1);
   return LSQED_OK;
 }
 
@@ -470,7 +483,6 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
   std::vector<double> T((size_t)m * m, 0.0), arrow;
   std::vector<double> beta(m + 1, 0.0);
   int k = 0;
-  double tnorm = 0.0;
   e->its = 0;
   e->nconv = 0;
   e->evals.clear();
@@ -482,16 +494,16 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
       if ((rc = launch_hv(e->h, col(j), col(j + 1), nullptr, 0, c.s))) return rc;
       std::vector<double> hcoef;
       double b;
-      if ((rc = cgs2(c, j + 1, col(j + 1), hcoef, b))) return rc;
+      bool lindep = false;
+      if ((rc = cgs2(c, j + 1, col(j + 1), hcoef, b, &lindep))) return rc;
       T[(size_t)j * m + j] = hcoef[j];
       if (j == k) {
         for (int i = 0; i < k; ++i) T[(size_t)i * m + k] = T[(size_t)k * m + i] = arrow[i];
       } else {
         T[(size_t)j * m + j - 1] = T[(size_t)(j - 1) * m + j] = beta[j - 1];
       }
-      tnorm = std::max(tnorm, std::fabs(hcoef[j]) + b);
       beta[j] = b;
-      if (b <= 1e-13 * tnorm || b == 0.0) {
+      if (lindep || (int64_t)(j + 1) >= D) {
         meff = j + 1;
         breakdown = true;
         break;

@@ -199,7 +199,8 @@ def parse_solver(path):
         m = re.match(r"TEST\((\w+),\s*(\w+)\)", s)
         if m:
             cur = dict(test=f"{m.group(1)}.{m.group(2)}", src=f"{rel(path)}:{ln_no}", J1=0.0, D1=0.0, J2=0.0,
-                       D2=0.0, builds=[], nev=None, ncv=None, mpd=None, eigs=[], tol=None)
+                       D2=0.0, builds=[], nev=None, ncv=None, mpd=None, eigs=[], tol=None, assert_gt=None,
+                       check_gt=None)
             cases.append(cur)
             refeigs = {}
             continue
@@ -228,6 +229,14 @@ def parse_solver(path):
         if m:
             cur["eigs"].append(float(m.group(1)))
             cur["tol"] = float(m.group(2))
+        m = re.search(r"ASSERT_GT\(nconv,\s*(\d+)\)", s)
+        if m:
+            cur["assert_gt"] = int(m.group(1))
+        m = re.search(r"if \(nconv > (\d+)\)", s)
+        if m:
+            cur["check_gt"] = int(m.group(1))
+        if re.search(r"for \(int i = 0; i < nconv; \+\+i\)", s):
+            cur["check_gt"] = 0
         m = re.search(r"ref_eigs\[(\d+)\]\s*=\s*(" + NUM + ")", s)
         if m:
             refeigs[int(m.group(1))] = float(m.group(2))
