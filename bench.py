@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lx", type=int, default=4)
     ap.add_argument("--ly", type=int, default=8)
+    ap.add_argument("--hv-kernel", type=int, default=0, help="0 auto, 1 generic, 2 row split, 3 specialised row split")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Sz-sector path here is built and measured on a single "
@@ -90,6 +91,8 @@ def main():
     h = Q.Hamiltonian(env, basis, lat, J1, D1, J2, D2)
     h.build_diag(env)
     h.build_off_diag(env)
+    if args.hv_kernel:
+        h.set_hv_kernel(args.hv_kernel)
     nflip = len(lat.nn_bonds[0]) + len(lat.nnn_bonds[0])
     B, D, nnz = algorithmic_bytes(N, N // 2, nflip)
     assert D == h.hamilt_size()
@@ -153,7 +156,8 @@ def main():
         "data": "synthetic (seeded uniform start vector)",
         "config": {"workload": f"Sz=0 H.v + 3-term Lanczos step, periodic square {lx}x{ly} (N={N}), "
                                f"J1=D1=1, J2=D2=0.5, D={D}", "D": D, "nnz_offdiag": nnz,
-                   "algorithmic_bytes_per_hv": B, "parallelism": "single GPU"},
+                   "algorithmic_bytes_per_hv": B, "parallelism": "single GPU",
+                   "hv_kernel": {0: "auto", 1: "generic", 2: "row split", 3: "row split (hipRTC)"}[args.hv_kernel]},
         "hv_ms": round(hv_ms, 3),
         "lanczos_steps_per_s": round(steps_per_s, 3),
         "lanczos_wall_s": round(wall, 3),

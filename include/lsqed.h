@@ -96,6 +96,15 @@ int lsqed_ham_set_fields(lsqed_ham* h, const double* hi);
  * on the device, must not alias. */
 int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
 
+/* H.v kernel choice: 0 = automatic (specialised row split when N <= 32 and
+ * hipRTC is available, else the row split, else the generic chunk-table
+ * kernel), 1 = generic chunk-table kernel, 2 = row-split kernel, 3 = row
+ * split specialised to the lattice (compiled once per bond structure with
+ * hipRTC).  No counterpart in the reference (whose operator is an assembled
+ * matrix); used by the tests to check every kernel and by the tuner.
+ * Setting LSQED_JIT=0 in the environment keeps automatic mode off hipRTC. */
+int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant);
+
 /* Diagonal of H exactly as the reference stores it (MatGetDiagonal,
  * tests/unit_tests/build_diag/chain_1d/test_build_diag60.cpp:29): entries the
  * reference never sets are 0.  d: D doubles (device). */

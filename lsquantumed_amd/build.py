@@ -16,7 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblsqed.so")
-SOURCES = ["hamiltonian.hip", "krylov.hip", "observables.hip", "batched.hip", "lattice.cpp"]
+SOURCES = ["hamiltonian.hip", "hv_split.hip", "krylov.hip", "observables.hip", "batched.hip", "lattice.cpp",
+           "hv_jit.cpp"]
 
 ARCH = os.environ.get("LSQED_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -30,7 +31,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 def _compile(src: str, obj: str) -> None:
     cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, "-c", src, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, *COMMON, "-x", "c++", "-c", src, "-o", obj]
+        rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+        cmd = [HIPCC, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -50,7 +52,7 @@ def build(verbose: bool = False) -> str:
         for f in futs:
             f.result()
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

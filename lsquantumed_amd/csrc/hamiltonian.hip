@@ -294,13 +294,42 @@ static int launch_hv_t(lsqed_ham* h, const double* x, double* y, double* dot_par
   return LSQED_OK;
 }
 
-// Public helper: y = H x, and (when dot_partials != null) per-block partial
-// sums of x.y into dot_partials[0..grid).  Returns the grid used via
-// *nblocks_dot semantics: callers size dot_partials with hv_grid().
-int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, int grid, hipStream_t s) {
+// Kernel choice for H.v: 3 = specialised row split (hipRTC), 2 = row split,
+// 1 = generic chunk table.
+static int hv_kind(lsqed_ham* h) {
+  if (h->hv_variant == 1) return 1;
+  if (h->hv_variant == 2) return 2;
+  if (!split_supported(h)) return 1;
+  if (h->hv_variant == 3) return 3;
+  if (jit_enabled() && !h->jit_failed && split_jit_fits(h)) {
+    if (jit_prepare(h) == LSQED_OK) return 3;
+    h->jit_failed = true;  // keep the unspecialised row split
+  }
+  return 2;
+}
+
+int hv_launch_grid(lsqed_ham* h) {
+  const int k = hv_kind(h);
+  if (k >= 2) {
+    if (split_plan_create(h) != LSQED_OK) return 0;
+    return split_grid(h);
+  }
+  return hv_grid(h, (h->D + kHvThreads - 1) / kHvThreads);
+}
+
+// y = H x, and (when dot_partials != null) per-block partial sums of x.y in
+// dot_partials[0 .. hv_launch_grid(h)).
+int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, hipStream_t s) {
   if (h->D == 0) return LSQED_OK;
+  const int k = hv_kind(h);
+  if (k == 3) {
+    int rc = split_plan_create(h);
+    if (rc) return rc;
+    return launch_hv_jit(h, x, y, dot_partials, split_grid(h), s);
+  }
+  if (k == 2) return launch_hv_split(h, x, y, dot_partials, 0, s);
   const int64_t ntiles = (h->D + kHvThreads - 1) / kHvThreads;
-  if (grid <= 0) grid = hv_grid(h, ntiles);
+  const int grid = hv_grid(h, ntiles);
   if (!h->tab.wide) {
     if (dot_partials) return launch_hv_t<uint32_t, uint32_t, 8, 4, true>(h, x, y, dot_partials, grid, s);
     return launch_hv_t<uint32_t, uint32_t, 8, 4, false>(h, x, y, nullptr, grid, s);
@@ -421,6 +450,7 @@ int lsqed_ham_destroy(lsqed_ham* h) {
   (void)hipFree(h->d_bonds_ordered);
   (void)hipFree(h->d_fields);
   (void)hipFree(h->d_scratch);
+  split_plan_destroy(h);
   delete h;
   return LSQED_OK;
 }
@@ -435,6 +465,7 @@ int lsqed_ham_set_build(lsqed_ham* h, int flags) {
     return fail(LSQED_ERR_ARG, "unknown build flag");
   if ((flags & LSQED_BUILD_DISORDER) && !h->d_fields)
     return fail(LSQED_ERR_STATE, "build_disorder needs fields (lsqed_ham_set_fields)");
+  if (flags != h->build) h->jit = nullptr;  // the specialised kernel bakes in the build flags
   h->build = flags;
   return LSQED_OK;
 }
@@ -443,12 +474,14 @@ int lsqed_ham_set_fields(lsqed_ham* h, const double* hi) {
   if (!h) return fail(LSQED_ERR_ARG, "null handle");
   if (!hi) {
     h->fields.clear();
+    if (h->build & LSQED_BUILD_DISORDER) h->jit = nullptr;
     h->build &= ~LSQED_BUILD_DISORDER;
     return LSQED_OK;
   }
   h->fields.assign(hi, hi + h->N);
   if (!h->d_fields) LSQED_HIP(hipMalloc((void**)&h->d_fields, h->N * sizeof(double)));
   LSQED_HIP(hipMemcpy(h->d_fields, hi, h->N * sizeof(double), hipMemcpyHostToDevice));
+  if (!(h->build & LSQED_BUILD_DISORDER)) h->jit = nullptr;
   h->build |= LSQED_BUILD_DISORDER;
   return LSQED_OK;
 }
@@ -456,7 +489,21 @@ int lsqed_ham_set_fields(lsqed_ham* h, const double* hi) {
 int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream) {
   if (!h || !x || !y) return fail(LSQED_ERR_ARG, "null argument");
   if (x == y) return fail(LSQED_ERR_ARG, "x and y must not alias");
-  return launch_hv(h, x, y, nullptr, 0, (hipStream_t)stream);
+  return launch_hv(h, x, y, nullptr, (hipStream_t)stream);
+}
+
+int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant) {
+  if (!h) return fail(LSQED_ERR_ARG, "null handle");
+  if (variant < 0 || variant > 3)
+    return fail(LSQED_ERR_ARG, "variant must be 0 (auto), 1 (generic), 2 (row split) or 3 (specialised row split)");
+  if (variant >= 2 && !split_supported(h)) return fail(LSQED_ERR_ARG, "the row-split kernels need N <= 32");
+  if (variant == 3) {
+    if (!split_jit_fits(h)) return fail(LSQED_ERR_ARG, "more than 64 bonds between high sites");
+    int rc = jit_prepare(h);
+    if (rc) return rc;
+  }
+  h->hv_variant = variant;
+  return LSQED_OK;
 }
 
 int lsqed_diag(lsqed_ham* h, double* d, lsqed_stream_t stream) {

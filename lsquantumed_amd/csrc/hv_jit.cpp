@@ -1,0 +1,449 @@
+// hv_jit.cpp -- lattice-specialised row-split H.v, compiled at run time.
+//
+// The row-split kernel of hv_split.hip reads its bond descriptors from kernel
+// arguments; with ~128 bonds the compiler hoists every decoded descriptor and
+// spills scalar registers.  Here the same algorithm is emitted as HIP source
+// with the bond classes as constexpr arrays (site indices, bond kinds, chunk
+// types become immediates, every bond loop fully unrolls), compiled once per
+// bond structure with hipRTC for gfx950 and cached for the process.  Couplings,
+// fields and build flags stay run-time arguments, so one compiled kernel serves
+// every realisation of a lattice.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "lsqed_internal.hpp"
+
+namespace lsqed {
+
+// Device code of the specialised kernel.  The generated prologue defines
+// N, NUP, L, NH, W1, the bond class arrays (LOW*, CR*, HI*) and their counts.
+static const char* kJitBody = R"JIT(
+constexpr int kThreads = 256;
+constexpr int kPerLane = 8;
+constexpr int kChunk = kThreads * kPerLane;
+
+struct JitArgs {
+  const double* x;
+  double* y;
+  double* dot;
+  const double* fields;
+  const unsigned short* U;
+  const unsigned short* RL0;
+  const unsigned short* RL1;
+  const unsigned int* binom;
+  const unsigned int* tasks;
+  int ntasks;
+  int offU[17];
+};
+
+template <bool DOT>
+__device__ __forceinline__ void hv_body(const JitArgs& a) {
+  __shared__ unsigned short s_RL0[256];
+  __shared__ unsigned short s_RL1[9 * W1];
+  __shared__ unsigned int s_binom[(N + 1) * (NUP + 1)];
+  __shared__ double s_f[N];
+  __shared__ unsigned int s_hbase[NHIGH + 1];
+  __shared__ double s_hcoef[NHIGH + 1];
+  __shared__ unsigned int s_cbase[NH + 1];
+  __shared__ unsigned int s_B;
+  __shared__ int s_nact, s_cnt0, s_cnt1;
+  __shared__ double s_dish;
+  __shared__ double s_red[kThreads / 64];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += kThreads) s_RL0[i] = a.RL0[i];
+  for (int i = tid; i < 9 * W1; i += kThreads) s_RL1[i] = a.RL1[i];
+  for (int i = tid; i < (N + 1) * (NUP + 1); i += kThreads) s_binom[i] = a.binom[i];
+  if (DIAG_MODE == 1)
+    for (int i = tid; i < N; i += kThreads) s_f[i] = a.fields[i];
+  __syncthreads();
+  auto C = [&](int n, int k) -> unsigned int { return (k < 0 || k > NUP) ? 0u : s_binom[n * (NUP + 1) + k]; };
+  auto rowbase = [&](unsigned int hh) -> unsigned int {
+    const int kl = NUP - __builtin_popcount(hh);
+    unsigned int b = 0;
+    int j = 0;
+    for (int q = 0; q < NH; ++q)
+      if ((hh >> q) & 1u) { ++j; b += C(L + q, kl + j); }
+    return b;
+  };
+  constexpr double jh[2] = {JH0, JH1};
+  constexpr int flip[2] = {FLIP0, FLIP1};
+  const long long G = gridDim.x;
+  const long long lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  const long long ta = lb * a.ntasks / G, tb = (lb + 1) * a.ntasks / G;
+  double dotacc = 0.0;
+  for (long long t = ta; t < tb; ++t) {
+    const unsigned int task = a.tasks[t];
+    const unsigned int h = task & 0xffffu;
+    const int kl = NUP - __builtin_popcount(h);
+    const int R = (int)C(L, kl);
+    const int i0 = (int)(task >> 16) * kChunk;
+    __syncthreads();
+    if (tid < NHIGH) {
+      const bool anti = __builtin_popcount(h & HIM[tid]) == 1;
+      s_hbase[tid] = (anti && flip[HIK[tid]]) ? rowbase(h ^ HIM[tid]) : 0xffffffffu;
+    } else if (tid >= 64 && tid < 64 + NH) {
+      s_cbase[tid - 64] = rowbase(h ^ (1u << (tid - 64)));
+    } else if (tid == 192) {
+      s_B = rowbase(h);
+      int c0 = 0, c1 = 0;
+#pragma unroll
+      for (int b = 0; b < NHIGH; ++b)
+        if (__builtin_popcount(h & HIM[b]) == 1) { if (HIK[b]) ++c1; else ++c0; }
+      s_cnt0 = c0;
+      s_cnt1 = c1;
+      double dh = 0.0;
+      if (DIAG_MODE == 1)
+        for (int q = 0; q < NH; ++q) dh = ((h >> q) & 1u) ? dh + s_f[L + q] : dh - s_f[L + q];
+      s_dish = dh;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int n = 0;
+      for (int b = 0; b < NHIGH; ++b)
+        if (s_hbase[b] != 0xffffffffu) { s_hbase[n] = s_hbase[b]; s_hcoef[n] = jh[HIK[b]]; ++n; }
+      s_nact = n;
+    }
+    __syncthreads();
+    const unsigned int B = __builtin_amdgcn_readfirstlane(s_B);
+    const int nact = __builtin_amdgcn_readfirstlane(s_nact);
+    const int cnt0 = __builtin_amdgcn_readfirstlane(s_cnt0), cnt1 = __builtin_amdgcn_readfirstlane(s_cnt1);
+    unsigned int cb[NH + 1];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) cb[q] = __builtin_amdgcn_readfirstlane(s_cbase[q]);
+    const unsigned short* Uk = a.U + a.offU[kl];
+#pragma unroll 1
+    for (int k = 0; k < kPerLane; ++k) {
+      const int i = i0 + tid + k * kThreads;
+      if (i >= R) break;
+      const unsigned int r = B + (unsigned int)i;
+      double acc = 0.0;
+      // high bonds: row h ^ m, same entry; loads issued in batches of 8
+      for (int b0 = 0; b0 < nact; b0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          v[u] = 0.0;
+          if (b0 + u < nact) v[u] = a.x[__builtin_amdgcn_readfirstlane(s_hbase[b0 + u]) + (unsigned int)i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (b0 + u < nact) acc += s_hcoef[b0 + u] * v[u];
+      }
+      const unsigned int l = Uk[i];
+      const unsigned int v0 = l & 255u;
+      const int k0 = __builtin_popcount(v0);
+      const unsigned int t1 = s_RL1[k0 * W1 + (l >> 8)];
+      const unsigned int t0 = (unsigned int)i - t1;
+      int cc0 = 0, cc1 = 0;
+      // cross bonds: row h ^ (1 << p), entry RL(l ^ (1 << q)); branch-free
+      // index selection (non-flipping lanes read their own entry) so that the
+      // gathers of a batch are all in flight together
+#pragma unroll
+      for (int b0 = 0; b0 < NCROSS; b0 += 8) {
+        double v[8];
+        unsigned int onm = 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int b = b0 + u;
+          if (b < NCROSS) {
+            const int q = CRQ[b], hp = CRP[b], kind = CRK[b];
+            const unsigned int anti = ((l >> q) ^ (h >> hp)) & 1u;
+            if (kind) cc1 += anti; else cc0 += anti;
+            const bool on = anti && flip[kind];
+            onm |= (on ? 1u : 0u) << u;
+            const unsigned int l2 = l ^ (1u << q);
+            unsigned int i2;
+            if (q < 8) {
+              const unsigned int w0 = l2 & 255u;
+              i2 = s_RL0[w0] + s_RL1[__builtin_popcount(w0) * W1 + (l2 >> 8)];
+            } else {
+              i2 = t0 + s_RL1[k0 * W1 + (l2 >> 8)];
+            }
+            const unsigned int idx = on ? cb[hp] + i2 : r;
+            v[u] = a.x[idx];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int b = b0 + u;
+          if (b < NCROSS) acc += (((onm >> u) & 1u) ? jh[CRK[b]] : 0.0) * v[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // low bonds: inside row h, entry RL(l ^ m)
+#pragma unroll
+      for (int b0 = 0; b0 < NLOW; b0 += 8) {
+        double v[8];
+        unsigned int onm = 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int b = b0 + u;
+          if (b < NLOW) {
+            const unsigned int m = LOWM[b];
+            const int kind = LOWK[b], type = LOWT[b];
+            const unsigned int anti = __builtin_popcount(l & m) == 1 ? 1u : 0u;
+            if (kind) cc1 += anti; else cc0 += anti;
+            const bool on = anti && flip[kind];
+            onm |= (on ? 1u : 0u) << u;
+            const unsigned int l2 = l ^ m;
+            unsigned int i2;
+            if (type == 0) {
+              i2 = s_RL0[l2 & 255u] + t1;
+            } else if (type == 1) {
+              i2 = t0 + s_RL1[k0 * W1 + (l2 >> 8)];
+            } else {
+              const unsigned int w0 = l2 & 255u;
+              i2 = s_RL0[w0] + s_RL1[__builtin_popcount(w0) * W1 + (l2 >> 8)];
+            }
+            const unsigned int idx = on ? B + i2 : r;
+            v[u] = a.x[idx];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int b = b0 + u;
+          if (b < NLOW) acc += (((onm >> u) & 1u) ? jh[LOWK[b]] : 0.0) * v[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      const int c0 = NB0 - 2 * (cnt0 + cc0);
+      const int c1 = NB1 - 2 * (cnt1 + cc1);
+      double d = 0.0;
+      if (DIAG_MODE == 0) {
+        if (DZ0 && c0 != 0) d = DQ0 * (double)c0;
+        if (DZ1 && c1 != 0) d = d + DQ1 * (double)c1;
+      } else if (DIAG_MODE == 1) {
+        double dis = 0.0;
+#pragma unroll
+        for (int q = 0; q < L; ++q) dis = ((l >> q) & 1u) ? dis + s_f[q] : dis - s_f[q];
+        dis += s_dish;
+        d = (DQ0 * (double)c0 + DQ1 * (double)c1) + 0.5 * dis;
+      }
+      const double xr = a.x[r];
+      const double yr = d * xr + acc;
+      a.y[r] = yr;
+      if (DOT) dotacc += xr * yr;
+    }
+  }
+  if (DOT) {
+    double v = dotacc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) s_red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) tot += s_red[w];
+      a.dot[blockIdx.x] = tot;
+    }
+  }
+}
+
+#ifndef WAVES_PER_EU
+#define WAVES_PER_EU 4
+#endif
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+void hv_jit_nodot(JitArgs a) { hv_body<false>(a); }
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+void hv_jit_dot(JitArgs a) { hv_body<true>(a); }
+)JIT";
+
+struct JitArgs {
+  const double* x;
+  double* y;
+  double* dot;
+  const double* fields;
+  const uint16_t* U;
+  const uint16_t* RL0;
+  const uint16_t* RL1;
+  const uint32_t* binom;
+  const uint32_t* tasks;
+  int ntasks;
+  int offU[17];
+};
+
+struct JitModule {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn_dot = nullptr, fn_nodot = nullptr;
+  bool ok = false;
+  std::string log;
+};
+
+static std::mutex g_jit_mu;
+static std::map<std::string, std::shared_ptr<JitModule>> g_jit_cache;
+
+template <typename T>
+static void emit_array(std::ostringstream& o, const char* name, const std::vector<T>& v, const char* type) {
+  o << "constexpr " << type << " " << name << "[" << std::max<size_t>(v.size(), 1) << "] = {";
+  if (v.empty()) o << "0";
+  for (size_t i = 0; i < v.size(); ++i) o << (i ? "," : "") << v[i];
+  o << "};\n";
+}
+
+static std::string dbl(double v) {
+  char b[64];
+  std::snprintf(b, sizeof(b), "%.17g", v);
+  std::string s(b);
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+  return s;
+}
+
+// Source of the kernel specialised to the bond classes and couplings of `h`
+// (row split with L = min(16, N) low bits, see hv_split.hip).  Fields stay a
+// run-time array, so disorder realisations share one compiled kernel.
+std::string jit_source(const lsqed_ham* h) {
+  const int N = h->N, nup = h->nup, L = std::min(16, N), NH = N - L;
+  std::vector<unsigned> lowm, him;
+  std::vector<int> lowk, lowt, crq, crp, crk, hik;
+  for (int kind = 0; kind < 2; ++kind)
+    for (size_t b = 0; b < h->bi[kind].size(); ++b) {
+      const int a = h->bi[kind][b], c = h->bj[kind][b];
+      const int lo = std::min(a, c), hi = std::max(a, c);
+      if (lo == hi) continue;
+      if (hi < L) {
+        lowm.push_back((1u << lo) | (1u << hi));
+        lowk.push_back(kind);
+        lowt.push_back(hi < 8 ? 0 : (lo >= 8 ? 1 : 2));
+      } else if (lo >= L) {
+        him.push_back((1u << (lo - L)) | (1u << (hi - L)));
+        hik.push_back(kind);
+      } else {
+        crq.push_back(lo);
+        crp.push_back(hi - L);
+        crk.push_back(kind);
+      }
+    }
+  std::ostringstream o;
+  o << "// generated by liblsqed (hv_jit.cpp)\n";
+  o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
+    << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
+  o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
+  const RowParams rp = make_row_params(h);
+  o << "constexpr double JH0 = " << dbl(rp.jh[0]) << ", JH1 = " << dbl(rp.jh[1]) << ", DQ0 = " << dbl(rp.dq[0])
+    << ", DQ1 = " << dbl(rp.dq[1]) << ";\n";
+  o << "constexpr int FLIP0 = " << rp.flip[0] << ", FLIP1 = " << rp.flip[1] << ", DZ0 = " << rp.dz_on[0]
+    << ", DZ1 = " << rp.dz_on[1] << ", DIAG_MODE = " << rp.diag_mode << ";\n";
+  o << "constexpr int NLOW = " << lowm.size() << ", NCROSS = " << crq.size() << ", NHIGH = " << him.size()
+    << ";\n";
+  emit_array(o, "LOWM", lowm, "unsigned");
+  emit_array(o, "LOWK", lowk, "int");
+  emit_array(o, "LOWT", lowt, "int");
+  emit_array(o, "CRQ", crq, "int");
+  emit_array(o, "CRP", crp, "int");
+  emit_array(o, "CRK", crk, "int");
+  emit_array(o, "HIM", him, "unsigned");
+  emit_array(o, "HIK", hik, "int");
+  o << kJitBody;
+  return o.str();
+}
+
+static std::shared_ptr<JitModule> jit_compile(const std::string& src) {
+  auto m = std::make_shared<JitModule>();
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "lsqed_hv_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    m->log = "hiprtcCreateProgram failed";
+    return m;
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  if (ls > 1) {
+    std::string log(ls, '\0');
+    hiprtcGetProgramLog(prog, &log[0]);
+    m->log = log;
+  }
+  if (rc != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return m;
+  }
+  size_t sz = 0;
+  hiprtcGetCodeSize(prog, &sz);
+  std::vector<char> code(sz);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess) {
+    m->log += "\nhipModuleLoadData failed";
+    return m;
+  }
+  if (hipModuleGetFunction(&m->fn_dot, m->mod, "hv_jit_dot") != hipSuccess ||
+      hipModuleGetFunction(&m->fn_nodot, m->mod, "hv_jit_nodot") != hipSuccess) {
+    m->log += "\nhipModuleGetFunction failed";
+    return m;
+  }
+  m->ok = true;
+  return m;
+}
+
+// Split-plan device tables are shared with hv_split.hip; fetched through this
+// accessor to keep SplitPlan private there.
+struct SplitTables {
+  const uint16_t *U, *RL0, *RL1;
+  const uint32_t *binom, *tasks;
+  int ntasks;
+  const int* offU;
+};
+int split_tables(lsqed_ham* h, SplitTables& t);
+
+bool jit_enabled() {
+  const char* e = getenv("LSQED_JIT");
+  return !(e && e[0] == '0');
+}
+
+int jit_prepare(lsqed_ham* h) {
+  if (h->jit) return LSQED_OK;
+  const std::string src = jit_source(h);
+  std::shared_ptr<JitModule> m;
+  {
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    auto it = g_jit_cache.find(src);
+    if (it != g_jit_cache.end()) {
+      m = it->second;
+    } else {
+      m = jit_compile(src);
+      g_jit_cache[src] = m;
+    }
+  }
+  if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the specialised H.v failed: " + m->log);
+  h->jit = m.get();
+  return LSQED_OK;
+}
+
+int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s) {
+  int rc = jit_prepare(h);
+  if (rc) return rc;
+  SplitTables t;
+  if ((rc = split_tables(h, t))) return rc;
+  JitModule* m = (JitModule*)h->jit;
+  JitArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.x = x;
+  a.y = y;
+  a.dot = dot_part;
+  a.fields = h->d_fields;
+  a.U = t.U;
+  a.RL0 = t.RL0;
+  a.RL1 = t.RL1;
+  a.binom = t.binom;
+  a.tasks = t.tasks;
+  a.ntasks = t.ntasks;
+  for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
+  void* args[] = {&a};
+  LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, args,
+                                  nullptr));
+  return LSQED_OK;
+}
+
+}  // namespace lsqed

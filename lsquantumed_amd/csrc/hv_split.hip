@@ -1,0 +1,444 @@
+// hv_split.hip -- row-split matrix-free H.v for N <= 32 (the fast path).
+//
+// The colex rank separates into a high and a low part.  Write a state as
+// s = (h << L) | l with L = min(16, N) low bits.  With kl = nup - popc(h),
+//   r(s) = B(h) + RL(l),
+//   B(h)  = sum_{j-th set bit of h at p} C(L + p, kl + j)   (row base)
+//   RL(l) = sum_{j-th set bit of l at q} C(q, j)            (index inside the row)
+// so all states sharing h form one contiguous "row" of C(L, kl) entries, in
+// the order of the low patterns.  Bonds fall in three classes:
+//   high  (both sites >= L): maps row h to row h ^ m entry by entry -- a
+//         wave-uniform test and a contiguous, coalesced stream of x;
+//   cross (one site on each side): maps row h to row h ^ (1<<p) in the
+//         neighbouring popcount class, index RL(l ^ (1<<q)); the up-to-three
+//         cross bonds of a high site share that target row, which stays in L2;
+//   low   (both sites < L): stays inside row h (L2 resident).
+// A work item is (row h, chunk of 2048 entries); each lane owns 8 entries
+// of the chunk.  The row bases of the chunk's targets are computed once per
+// item, so the per-entry work is table lookups on 16-bit patterns (LDS) and
+// FP64 gathers/FMAs.  Same row semantics as hamiltonian.hip's generic kernel
+// (the parity reference for H.v is the oracle's CSR).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lsqed_internal.hpp"
+
+namespace lsqed {
+
+constexpr int kSplitThreads = 256;
+constexpr int kSplitPerLane = 8;
+constexpr int kSplitChunk = kSplitThreads * kSplitPerLane;  // 2048 entries per work item
+constexpr int kMaxClass = 64;                               // bonds per class
+
+struct SplitParams {
+  const double* fields;
+  const uint16_t* U;      // low patterns per popcount class, colex order
+  const uint16_t* RL0;    // 256: rank contribution of the low byte
+  const uint16_t* RL1;    // 9 x W1: contribution of bits 8..L-1, by popcount of the low byte
+  const uint32_t* binom;  // C(n, k), n <= N, k <= nup: [(n)*(nup+1) + k]
+  const uint32_t* tasks;  // h | chunk << 16
+  int N, nup, L, NH, ntasks, w1;  // w1 = 1 << max(L-8, 0)
+  int nlow, ncross, nhigh;
+  int offU[17];
+  uint32_t low[kMaxClass];    // mask (16 bits) | kind << 16 | type << 17 (0: byte 0, 1: byte 1, 2: split)
+  uint32_t cross[kMaxClass];  // q | p << 4 | kind << 8
+  uint32_t high[kMaxClass];   // mask on h (16 bits) | kind << 16
+  double jh[2], dq[2];
+  int flip[2], dz_on[2], nbk[2];
+  int diag_mode;
+};
+
+struct SplitPlan {
+  SplitParams p;
+  uint16_t* d_U = nullptr;
+  uint16_t* d_RL0 = nullptr;
+  uint16_t* d_RL1 = nullptr;
+  uint32_t* d_binom = nullptr;
+  uint32_t* d_tasks = nullptr;
+  int ntasks = 0;
+  size_t binom_n = 0;
+};
+
+// -------------------------------------------------------------- the kernel --
+template <bool DOT>
+__global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, const double* __restrict__ x,
+                                                                 double* __restrict__ y,
+                                                                 double* __restrict__ dot_part) {
+  __shared__ uint16_t s_RL0[256];
+  __shared__ uint16_t s_RL1[9 * 256];
+  __shared__ uint32_t s_binom[33 * 33];
+  __shared__ double s_f[64];
+  __shared__ uint32_t s_hbase[kMaxClass];
+  __shared__ double s_hcoef[kMaxClass];
+  __shared__ uint32_t s_cbase[16];
+  __shared__ int s_nact;
+  __shared__ uint32_t s_B;
+  __shared__ int s_cnt[2];
+  __shared__ double s_dish;
+  __shared__ double s_red[8];
+
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += kSplitThreads) s_RL0[i] = p.RL0[i];
+  for (int i = tid; i < 9 * p.w1; i += kSplitThreads) s_RL1[i] = p.RL1[i];
+  const int nb = (p.N + 1) * (p.nup + 1);
+  for (int i = tid; i < nb; i += kSplitThreads) s_binom[i] = p.binom[i];
+  if (p.diag_mode == 1)
+    for (int i = tid; i < p.N; i += kSplitThreads) s_f[i] = p.fields[i];
+  __syncthreads();
+
+  const int K1 = p.nup + 1;
+  auto C = [&](int n, int k) -> uint32_t { return (k < 0 || k > p.nup) ? 0u : s_binom[n * K1 + k]; };
+  // row base of high pattern hh
+  auto rowbase = [&](uint32_t hh) -> uint32_t {
+    const int kl = p.nup - __builtin_popcount(hh);
+    uint32_t b = 0;
+    int j = 0;
+    for (int q = 0; q < p.NH; ++q)
+      if ((hh >> q) & 1u) {
+        ++j;
+        b += C(p.L + q, kl + j);
+      }
+    return b;
+  };
+
+  // XCD-aware contiguous task ranges
+  const int64_t G = gridDim.x;
+  const int64_t lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  const int64_t ta = lb * p.ntasks / G, tb = (lb + 1) * p.ntasks / G;
+  double dotacc = 0.0;
+
+  for (int64_t t = ta; t < tb; ++t) {
+    const uint32_t task = p.tasks[t];
+    const uint32_t h = task & 0xffffu;
+    const int chunk = (int)(task >> 16);
+    const int kh = __builtin_popcount(h);
+    const int kl = p.nup - kh;
+    const int R = (int)C(p.L, kl);
+    const int i0 = chunk * kSplitChunk;
+    __syncthreads();  // previous item's LDS reads done
+    if (tid < p.nhigh) {
+      const uint32_t hd = p.high[tid];
+      const uint32_t m = hd & 0xffffu;
+      const bool anti = __builtin_popcount(h & m) == 1;
+      s_hbase[tid] = anti ? rowbase(h ^ m) : 0xffffffffu;
+    } else if (tid >= 64 && tid < 64 + p.NH) {
+      s_cbase[tid - 64] = rowbase(h ^ (1u << (tid - 64)));
+    } else if (tid == 128) {
+      s_B = rowbase(h);
+      int c0 = 0, c1 = 0;
+      for (int b = 0; b < p.nhigh; ++b) {
+        const uint32_t hd = p.high[b];
+        if (__builtin_popcount(h & (hd & 0xffffu)) == 1) {
+          if ((hd >> 16) & 1) ++c1; else ++c0;
+        }
+      }
+      s_cnt[0] = c0;
+      s_cnt[1] = c1;
+      double dh = 0.0;
+      if (p.diag_mode == 1)
+        for (int q = 0; q < p.NH; ++q) dh = ((h >> q) & 1u) ? dh + s_f[p.L + q] : dh - s_f[p.L + q];
+      s_dish = dh;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int n = 0;
+      for (int b = 0; b < p.nhigh; ++b) {
+        const uint32_t base = s_hbase[b];
+        const int kind = (p.high[b] >> 16) & 1;
+        if (base != 0xffffffffu && p.flip[kind]) {
+          s_hbase[n] = base;
+          s_hcoef[n] = p.jh[kind];
+          ++n;
+        }
+      }
+      s_nact = n;
+    }
+    __syncthreads();
+    const uint32_t B = s_B;
+    const int nact = s_nact;
+
+    const uint16_t* Uk = p.U + p.offU[kl];
+#pragma unroll 1
+    for (int k = 0; k < kSplitPerLane; ++k) {
+      const int i = i0 + tid + k * kSplitThreads;
+      if (i >= R) break;
+      // high bonds: the whole row h ^ m, entry i (coalesced across lanes)
+      double a = 0.0;
+      for (int b = 0; b < nact; ++b) {
+        const uint32_t base = __builtin_amdgcn_readfirstlane(s_hbase[b]);
+        a += s_hcoef[b] * x[base + (uint32_t)i];
+      }
+      // cross and low bonds
+      const uint32_t l = Uk[i];
+      const uint32_t lb0 = l & 255u;
+      const int k0 = __builtin_popcount(lb0);
+      const uint32_t t1 = s_RL1[k0 * p.w1 + (l >> 8)];
+      const uint32_t t0 = (uint32_t)i - t1;
+      int cc0 = 0, cc1 = 0;
+#pragma unroll
+      for (int b = 0; b < kMaxClass; ++b) {
+        if (b < p.ncross) {
+          const uint32_t cd = p.cross[b];
+          const int q = cd & 15, hp = (cd >> 4) & 15, kind = (cd >> 8) & 1;
+          const uint32_t bq = (l >> q) & 1u, bp = (h >> hp) & 1u;
+          if (bq != bp) {
+            if (kind) ++cc1; else ++cc0;
+            if (p.flip[kind]) {
+              const uint32_t l2 = l ^ (1u << q);
+              uint32_t i2;
+              if (q < 8) {
+                const uint32_t v0 = l2 & 255u;
+                i2 = s_RL0[v0] + s_RL1[__builtin_popcount(v0) * p.w1 + (l2 >> 8)];
+              } else {
+                i2 = t0 + s_RL1[k0 * p.w1 + (l2 >> 8)];
+              }
+              a += p.jh[kind] * x[s_cbase[hp] + i2];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < kMaxClass; ++b) {
+        if (b < p.nlow) {
+          const uint32_t ld = p.low[b];
+          const uint32_t m = ld & 0xffffu;
+          const int kind = (ld >> 16) & 1, type = (ld >> 17) & 3;
+          if (__builtin_popcount(l & m) == 1) {
+            if (kind) ++cc1; else ++cc0;
+            if (p.flip[kind]) {
+              const uint32_t l2 = l ^ m;
+              uint32_t i2;
+              if (type == 0)
+                i2 = s_RL0[l2 & 255u] + t1;
+              else if (type == 1)
+                i2 = t0 + s_RL1[k0 * p.w1 + (l2 >> 8)];
+              else {
+                const uint32_t v0 = l2 & 255u;
+                i2 = s_RL0[v0] + s_RL1[__builtin_popcount(v0) * p.w1 + (l2 >> 8)];
+              }
+              a += p.jh[kind] * x[B + i2];
+            }
+          }
+        }
+      }
+      // diagonal
+      const int c0 = p.nbk[0] - 2 * (s_cnt[0] + cc0);
+      const int c1 = p.nbk[1] - 2 * (s_cnt[1] + cc1);
+      double d = 0.0;
+      if (p.diag_mode == 0) {
+        if (p.dz_on[0] && c0 != 0) d = p.dq[0] * (double)c0;
+        if (p.dz_on[1] && c1 != 0) d = d + p.dq[1] * (double)c1;
+      } else if (p.diag_mode == 1) {
+        double dis = 0.0;
+        for (int q = 0; q < p.L; ++q) dis = ((l >> q) & 1u) ? dis + s_f[q] : dis - s_f[q];
+        dis += s_dish;
+        d = (p.dq[0] * (double)c0 + p.dq[1] * (double)c1) + 0.5 * dis;
+      }
+      const uint32_t r = B + (uint32_t)i;
+      const double xr = x[r];
+      const double yr = d * xr + a;
+      y[r] = yr;
+      if (DOT) dotacc += xr * yr;
+    }
+  }
+  if (DOT) {
+    double v = dotacc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) s_red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < kSplitThreads / 64; ++w) tot += s_red[w];
+      dot_part[blockIdx.x] = tot;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- host --
+static int count_bits(uint32_t v) { return __builtin_popcount(v); }
+
+bool split_supported(const lsqed_ham* h) {
+  if (h->N > 32 || h->N < 2) return false;
+  if ((int)h->bi[0].size() + (int)h->bi[1].size() > 3 * kMaxClass) return false;
+  return true;
+}
+
+void split_plan_destroy(lsqed_ham* h) {
+  SplitPlan* sp = (SplitPlan*)h->split;
+  if (!sp) return;
+  (void)hipFree(sp->d_U);
+  (void)hipFree(sp->d_RL0);
+  (void)hipFree(sp->d_RL1);
+  (void)hipFree(sp->d_binom);
+  (void)hipFree(sp->d_tasks);
+  delete sp;
+  h->split = nullptr;
+}
+
+int split_plan_create(lsqed_ham* h) {
+  if (h->split) return LSQED_OK;
+  if (!split_supported(h)) return fail(LSQED_ERR_ARG, "row split needs N <= 32");
+  const int N = h->N, nup = h->nup;
+  const int L = std::min(16, N), NH = N - L;
+  SplitPlan* sp = new SplitPlan;
+  SplitParams& p = sp->p;
+  std::memset(&p, 0, sizeof(p));
+  p.N = N;
+  p.nup = nup;
+  p.L = L;
+  p.NH = NH;
+  p.w1 = 1 << std::max(L - 8, 0);
+  // low patterns by popcount class, numeric (= colex) order
+  std::vector<uint16_t> U;
+  for (int k = 0; k <= L; ++k) {
+    p.offU[k] = (int)U.size();
+    for (uint32_t v = 0; v < (1u << L); ++v)
+      if (count_bits(v) == k) U.push_back((uint16_t)v);
+  }
+  std::vector<uint16_t> RL0(256, 0), RL1((size_t)9 * p.w1, 0);
+  for (int v = 0; v < 256; ++v) {
+    if (L < 8 && (v >> L)) continue;
+    uint32_t r = 0;
+    int j = 0;
+    for (int q = 0; q < 8; ++q)
+      if ((v >> q) & 1) r += (uint32_t)binom_u64(q, ++j);
+    RL0[v] = (uint16_t)r;
+  }
+  for (int k = 0; k <= 8; ++k)
+    for (int v = 0; v < p.w1; ++v) {
+      uint32_t r = 0;
+      int j = 0;
+      for (int q = 0; q < L - 8; ++q)
+        if ((v >> q) & 1) r += (uint32_t)binom_u64(8 + q, k + (++j));
+      RL1[(size_t)k * p.w1 + v] = (uint16_t)r;
+    }
+  std::vector<uint32_t> bin((size_t)(N + 1) * (nup + 1));
+  for (int n = 0; n <= N; ++n)
+    for (int k = 0; k <= nup; ++k) bin[(size_t)n * (nup + 1) + k] = (uint32_t)binom_u64(n, k);
+  // bond classes
+  for (int kind = 0; kind < 2; ++kind) {
+    p.nbk[kind] = (int)h->bi[kind].size();
+    for (size_t b = 0; b < h->bi[kind].size(); ++b) {
+      int a = h->bi[kind][b], c = h->bj[kind][b];
+      int lo = std::min(a, c), hi = std::max(a, c);
+      if (lo == hi) continue;  // a self bond is always parallel: diagonal only (counted via nbk)
+      if (hi < L) {
+        const int type = hi < 8 ? 0 : (lo >= 8 ? 1 : 2);
+        p.low[p.nlow++] = ((1u << lo) | (1u << hi)) | ((uint32_t)kind << 16) | ((uint32_t)type << 17);
+      } else if (lo >= L) {
+        p.high[p.nhigh++] = ((1u << (lo - L)) | (1u << (hi - L))) | ((uint32_t)kind << 16);
+      } else {
+        p.cross[p.ncross++] = (uint32_t)lo | ((uint32_t)(hi - L) << 4) | ((uint32_t)kind << 8);
+      }
+      if (p.nlow > kMaxClass || p.nhigh > kMaxClass || p.ncross > kMaxClass) {
+        delete sp;
+        return fail(LSQED_ERR_ARG, "too many bonds of one class for the row split");
+      }
+    }
+  }
+  // work items: (h, chunk) for every row with a valid low class
+  std::vector<uint32_t> tasks;
+  for (uint32_t hh = 0; hh < (1u << NH); ++hh) {
+    const int kl = nup - count_bits(hh);
+    if (kl < 0 || kl > L) continue;
+    const int64_t R = (int64_t)binom_u64(L, kl);
+    for (int64_t c = 0; c * kSplitChunk < R; ++c) tasks.push_back(hh | ((uint32_t)c << 16));
+  }
+  sp->ntasks = (int)tasks.size();
+  p.ntasks = sp->ntasks;
+  auto up = [&](void** dst, const void* src, size_t bytes) -> bool {
+    if (hipMalloc(dst, std::max<size_t>(bytes, 4)) != hipSuccess) return false;
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up((void**)&sp->d_U, U.data(), U.size() * 2) || !up((void**)&sp->d_RL0, RL0.data(), RL0.size() * 2) ||
+      !up((void**)&sp->d_RL1, RL1.data(), RL1.size() * 2) || !up((void**)&sp->d_binom, bin.data(), bin.size() * 4) ||
+      !up((void**)&sp->d_tasks, tasks.data(), tasks.size() * 4)) {
+    h->split = sp;
+    split_plan_destroy(h);
+    return fail(LSQED_ERR_HIP, "row-split table upload failed");
+  }
+  p.U = sp->d_U;
+  p.RL0 = sp->d_RL0;
+  p.RL1 = sp->d_RL1;
+  p.binom = sp->d_binom;
+  p.tasks = sp->d_tasks;
+  h->split = sp;
+  return LSQED_OK;
+}
+
+struct SplitTables {
+  const uint16_t *U, *RL0, *RL1;
+  const uint32_t *binom, *tasks;
+  int ntasks;
+  const int* offU;
+};
+
+int split_tables(lsqed_ham* h, SplitTables& t) {
+  int rc = split_plan_create(h);
+  if (rc) return rc;
+  SplitPlan* sp = (SplitPlan*)h->split;
+  t.U = sp->d_U;
+  t.RL0 = sp->d_RL0;
+  t.RL1 = sp->d_RL1;
+  t.binom = sp->d_binom;
+  t.tasks = sp->d_tasks;
+  t.ntasks = sp->ntasks;
+  t.offU = sp->p.offU;
+  return LSQED_OK;
+}
+
+// The specialised kernel handles up to 64 high bonds (one prologue lane each).
+bool split_jit_fits(const lsqed_ham* h) {
+  if (!split_supported(h)) return false;
+  const int L = std::min(16, h->N);
+  int nhigh = 0;
+  for (int k = 0; k < 2; ++k)
+    for (size_t b = 0; b < h->bi[k].size(); ++b)
+      if (std::min(h->bi[k][b], h->bj[k][b]) >= L && h->bi[k][b] != h->bj[k][b]) ++nhigh;
+  return nhigh <= 64;
+}
+
+int split_grid(const lsqed_ham* h) {
+  const SplitPlan* sp = (const SplitPlan*)h->split;
+  int cus = 256;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  int64_t g = (int64_t)cus * 8;
+  g = std::min<int64_t>(g, std::max(sp ? sp->ntasks : 1, 1));
+  g = ((g + 7) / 8) * 8;
+  return (int)g;
+}
+
+int launch_hv_split(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s) {
+  int rc = split_plan_create(h);
+  if (rc) return rc;
+  SplitPlan* sp = (SplitPlan*)h->split;
+  SplitParams p = sp->p;
+  const RowParams rp = make_row_params(h);
+  p.fields = h->d_fields;
+  for (int k = 0; k < 2; ++k) {
+    p.jh[k] = rp.jh[k];
+    p.dq[k] = rp.dq[k];
+    p.flip[k] = rp.flip[k];
+    p.dz_on[k] = rp.dz_on[k];
+  }
+  p.diag_mode = rp.diag_mode;
+  if (grid <= 0) grid = split_grid(h);
+  if (dot_part)
+    hipLaunchKernelGGL(hv_split_kernel<true>, dim3(grid), dim3(kSplitThreads), 0, s, p, x, y, dot_part);
+  else
+    hipLaunchKernelGGL(hv_split_kernel<false>, dim3(grid), dim3(kSplitThreads), 0, s, p, x, y, nullptr);
+  LSQED_HIP(hipGetLastError());
+  return LSQED_OK;
+}
+
+}  // namespace lsqed

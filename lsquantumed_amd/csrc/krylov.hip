@@ -20,7 +20,6 @@
 
 namespace lsqed {
 
-int hv_grid(const lsqed_ham* h, int64_t ntiles);
 
 constexpr int kThreads = 256;
 constexpr int kRedBlocks = 1024;  // partial sums per reduction
@@ -491,7 +490,7 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
     int meff = m;
     bool breakdown = false;
     for (int j = k; j < m; ++j) {
-      if ((rc = launch_hv(e->h, col(j), col(j + 1), nullptr, 0, c.s))) return rc;
+      if ((rc = launch_hv(e->h, col(j), col(j + 1), nullptr, c.s))) return rc;
       std::vector<double> hcoef;
       double b;
       bool lindep = false;
@@ -571,7 +570,7 @@ int lsqed_eps_solution(lsqed_eps* e, int i, double* lambda, double* x, double* e
   if (error) {
     // EPSComputeError(EPS_ERROR_RELATIVE): ||H x - lambda x|| / |lambda|
     double* y = e->V + (int64_t)e->ncv * D;
-    int rc = launch_hv(e->h, vi, y, nullptr, 0, c.s);
+    int rc = launch_hv(e->h, vi, y, nullptr, c.s);
     if (rc) return rc;
     const int G = red_blocks(D);
     hipLaunchKernelGGL(k_resid, dim3(G), dim3(kThreads), 0, c.s, y, vi, lam, D, e->part);
@@ -589,15 +588,15 @@ int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, doub
   if (!h || !q || !p || !y) return fail(LSQED_ERR_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   const int64_t D = h->D;
-  const int64_t ntiles = (D + 255) / 256;
-  const int G = hv_grid(h, ntiles);
+  const int G = hv_launch_grid(h);
+  if (G <= 0) return fail(LSQED_ERR_HIP, "H.v plan setup failed");
   const int R = red_blocks(D);
   int rc = ensure_scratch(h, (size_t)G + R + 8);
   if (rc) return rc;
   double* dpart = h->d_scratch;
   double* upart = h->d_scratch + G;
   double* outv = h->d_scratch + G + R;
-  if ((rc = launch_hv(h, q, y, dpart, G, s))) return rc;
+  if ((rc = launch_hv(h, q, y, dpart, s))) return rc;
   double qy = 0.0;
   if ((rc = finish_reduce(h, dpart, G, 1, outv, &qy, s))) return rc;
   const double a = sq * sq * qy;

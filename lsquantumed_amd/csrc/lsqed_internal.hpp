@@ -75,6 +75,10 @@ struct lsqed_ham {
   double* d_scratch = nullptr;   // reduction partials
   size_t scratch_doubles = 0;
   int device = 0;
+  int hv_variant = 0;          // 0 auto, 1 generic chunk-table, 2 row split, 3 row split specialised (hipRTC)
+  void* split = nullptr;       // row-split plan (hv_split.hip), built on first use
+  void* jit = nullptr;         // specialised row-split module (hv_jit.cpp), cached per bond structure
+  bool jit_failed = false;
 };
 
 namespace lsqed {
@@ -113,7 +117,23 @@ RowParams make_row_params(const lsqed_ham* h);
 int ensure_scratch(lsqed_ham* h, size_t doubles);
 
 // Launch helpers implemented in hamiltonian.hip (used by krylov.hip).
-int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, int nblocks_dot, hipStream_t s);
+// hv_launch_grid() is the number of blocks launch_hv() uses, i.e. the number
+// of partial sums it writes when dot_partials != null.
+int hv_launch_grid(lsqed_ham* h);
+int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, hipStream_t s);
+
+// Row-split kernel (hv_split.hip).
+bool split_supported(const lsqed_ham* h);
+int split_plan_create(lsqed_ham* h);
+void split_plan_destroy(lsqed_ham* h);
+int split_grid(const lsqed_ham* h);
+int launch_hv_split(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
+bool split_jit_fits(const lsqed_ham* h);
+
+// Specialised row-split kernel (hv_jit.cpp).
+bool jit_enabled();
+int jit_prepare(lsqed_ham* h);
+int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
 
 // Reductions (krylov.hip)
 int device_dot(lsqed_ham* h, const double* x, const double* y, int64_t n, double* out_host, hipStream_t s);

@@ -230,6 +230,13 @@ class Hamiltonian:
         return self.Delta2
 
     # --- device operations ------------------------------------------------
+    HV_AUTO, HV_GENERIC, HV_SPLIT, HV_SPLIT_JIT = 0, 1, 2, 3
+
+    def set_hv_kernel(self, variant: int) -> None:
+        """Select the H.v kernel (0 auto, 1 generic chunk table, 2 row split,
+        3 row split specialised to this lattice with hipRTC)."""
+        check(lib().lsqed_ham_set_hv_kernel(self.hamilt, int(variant)))
+
     def matvec(self, x, y=None, stream=None):
         """y = H x on the GPU (x, y: float64 CUDA tensors of length D)."""
         import torch

@@ -147,12 +147,14 @@ def test_diag_fixtures_bitexact(O, torch, literals, refdata):
 
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}m{c[2]}" for c in CASES])
 @pytest.mark.parametrize("fields", [False, True])
-def test_hv_matches_oracle_csr(O, torch, case, fields):
+@pytest.mark.parametrize("kernel", [1, 2, 3], ids=["generic", "split", "split_jit"])
+def test_hv_matches_oracle_csr(O, torch, case, fields, kernel):
     lat, N, mag, J1, D1, J2, D2, lx, ly = case
-    say("hv", case, fields)
+    say("hv", case, fields, kernel)
     hi = np.random.default_rng(N).uniform(-2, 2, N) if fields else None
     builds = ("build_diag", "build_off_diag", "build_disorder") if fields else ("build_diag", "build_off_diag")
     h = make(lat, N, mag, J1, D1, J2, D2, lx, ly, builds=builds, hi=hi)
+    h.set_hv_kernel(kernel)
     m = oracle_model(O, lat, N, mag, J1, D1, J2, D2, lx, ly)
     A = m.csr(hi, flags_of(O, builds))
     D = h.hamilt_size()
@@ -181,11 +183,13 @@ def test_build_flags(O, torch):
 
 
 @pytest.mark.parametrize("lat,N,lx,ly,J2", [("square", 24, 4, 6, 0.5), ("chain", 22, 0, 0, 0.2)])
-def test_hv_config_sizes_matfree(O, torch, lat, N, lx, ly, J2):
+@pytest.mark.parametrize("kernel", [1, 2, 3], ids=["generic", "split", "split_jit"])
+def test_hv_config_sizes_matfree(O, torch, lat, N, lx, ly, J2, kernel):
     """Full-size H.v of the 4x6 J1-J2 lattice and the N=22 chain against the
     oracle's matrix-free product (same row semantics, exact binomials)."""
-    say("hv config", lat, N)
+    say("hv config", lat, N, kernel)
     h = make(lat, N, 0, 1.0, 1.0, J2, J2, lx, ly)
+    h.set_hv_kernel(kernel)
     m = oracle_model(O, lat, N, 0, 1.0, 1.0, J2, J2, lx, ly)
     D = h.hamilt_size()
     x = np.random.default_rng(5).uniform(-1, 1, D)
@@ -194,11 +198,13 @@ def test_hv_config_sizes_matfree(O, torch, lat, N, lx, ly, J2):
     assert np.max(np.abs(y - yref)) <= 1e-13 * np.abs(yref).max()
 
 
-def test_hv_4x8_properties_and_rows(O, torch):
+@pytest.mark.parametrize("kernel", [1, 2, 3], ids=["generic", "split", "split_jit"])
+def test_hv_4x8_properties_and_rows(O, torch, kernel):
     """4x8 J1-J2 (D = 601,080,390): rows sampled across the sector against
     the oracle's matrix-free rows, symmetry <x, H y> = <H x, y>, linearity."""
-    say("hv 4x8")
+    say("hv 4x8", kernel)
     h = make("square", 32, 0, 1.0, 1.0, 0.5, 0.5, 4, 8)
+    h.set_hv_kernel(kernel)
     m = oracle_model(O, "square", 32, 0, 1.0, 1.0, 0.5, 0.5, 4, 8)
     D = h.hamilt_size()
     assert D == 601080390
