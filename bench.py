@@ -33,6 +33,18 @@ def algorithmic_bytes(N, nup, n_bonds_flip):
     return 8 * (2 * D + nnz), D, nnz
 
 
+def measured_traffic(kernel_name, lx, ly):
+    """HBM bytes per H.v launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated against k_lanczos_update's known
+    traffic); null when no measurement of this kernel and lattice exists."""
+    path = os.path.join(ROOT, "profiles", f"hv_traffic_{lx}x{ly}.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)["kernels"][kernel_name]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(h_args, target_s=12.0):
     """Oracle matrix-free H.v rows/s on this host, scaled to GB/s of B_Hv."""
     import numpy as np
@@ -93,6 +105,7 @@ def main():
     h.build_off_diag(env)
     if args.hv_kernel:
         h.set_hv_kernel(args.hv_kernel)
+    kernel_name = {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()]
     nflip = len(lat.nn_bonds[0]) + len(lat.nnn_bonds[0])
     B, D, nnz = algorithmic_bytes(N, N // 2, nflip)
     assert D == h.hamilt_size()
@@ -157,12 +170,12 @@ def main():
         "config": {"workload": f"Sz=0 H.v + 3-term Lanczos step, periodic square {lx}x{ly} (N={N}), "
                                f"J1=D1=1, J2=D2=0.5, D={D}", "D": D, "nnz_offdiag": nnz,
                    "algorithmic_bytes_per_hv": B, "parallelism": "single GPU",
-                   "hv_kernel": {0: "auto", 1: "generic", 2: "row split", 3: "row split (hipRTC)"}[args.hv_kernel]},
+                   "hv_kernel": kernel_name},
         "hv_ms": round(hv_ms, 3),
         "lanczos_steps_per_s": round(steps_per_s, 3),
         "lanczos_wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(hv_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name, lx, ly)},
         "cpu_baseline": cpu,
         "alpha_last": alphas[-1],
     }

@@ -104,6 +104,8 @@ int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
  * matrix); used by the tests to check every kernel and by the tuner.
  * Setting LSQED_JIT=0 in the environment keeps automatic mode off hipRTC. */
 int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant);
+/* The kernel lsqed_hv will use (1, 2 or 3), resolving automatic mode. */
+int lsqed_ham_get_hv_kernel(lsqed_ham* h);
 
 /* Diagonal of H exactly as the reference stores it (MatGetDiagonal,
  * tests/unit_tests/build_diag/chain_1d/test_build_diag60.cpp:29): entries the

@@ -57,6 +57,7 @@ SIGNATURES = {
     "lsqed_ham_set_fields": (_i32, [_vp, _vp]),
     "lsqed_hv": (_i32, [_vp, _vp, _vp, _vp]),
     "lsqed_ham_set_hv_kernel": (_i32, [_vp, _i32]),
+    "lsqed_ham_get_hv_kernel": (_i32, [_vp]),
     "lsqed_diag": (_i32, [_vp, _vp, _vp]),
     "lsqed_export_cols": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _vp]),
     "lsqed_unrank": (_i32, [_vp, _i64, _i64, _vp, _vp]),

@@ -492,6 +492,8 @@ int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream) {
   return launch_hv(h, x, y, nullptr, (hipStream_t)stream);
 }
 
+int lsqed_ham_get_hv_kernel(lsqed_ham* h) { return h ? hv_kind(h) : 0; }
+
 int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant) {
   if (!h) return fail(LSQED_ERR_ARG, "null handle");
   if (variant < 0 || variant > 3)

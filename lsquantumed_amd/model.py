@@ -237,6 +237,10 @@ class Hamiltonian:
         3 row split specialised to this lattice with hipRTC)."""
         check(lib().lsqed_ham_set_hv_kernel(self.hamilt, int(variant)))
 
+    def hv_kernel(self) -> int:
+        """The kernel matvec uses (1 generic, 2 row split, 3 specialised row split)."""
+        return int(lib().lsqed_ham_get_hv_kernel(self.hamilt))
+
     def matvec(self, x, y=None, stream=None):
         """y = H x on the GPU (x, y: float64 CUDA tensors of length D)."""
         import torch
