@@ -92,6 +92,13 @@ int lsqed_ham_set_build(lsqed_ham* h, int flags);
  * hi == NULL clears the fields and the flag. */
 int lsqed_ham_set_fields(lsqed_ham* h, const double* hi);
 
+/* New couplings (J1, D1, J2, D2) for an existing model, as if the
+ * Hamiltonian<L>(env, basis, lattice, J1, D1, J2, D2) constructor (tmpl:69-91)
+ * had been called with them; lattice, sector and build flags are kept.  Used
+ * by the differentiable E0 (finite differences in J) instead of rebuilding
+ * the handle. */
+int lsqed_ham_set_couplings(lsqed_ham* h, double J1, double D1, double J2, double D2);
+
 /* y = H x (the MatMult of EPSSolve, src/solver.cpp:45).  x and y: D doubles
  * on the device, must not alias. */
 int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
@@ -185,6 +192,61 @@ int lsqed_obs_sz(lsqed_ham* h, const double* psi, double* out, lsqed_stream_t st
  * column-major D1-vectors (device).  out: nev*N doubles (host). */
 int lsqed_obs_dsf(lsqed_ham* h0, lsqed_ham* h1, const double* psi0, const double* evecs, int nev, double* out,
                   lsqed_stream_t stream);
+
+/* ------------------------------------------------------- batched solver -- */
+/* Ground states of many models at once: the loop of examples/
+ * main_disorder.cpp:82-120 (per realisation build_disorder, SolverInit with
+ * nev = 1, solve, solution) as one batched three-term Lanczos.  The models
+ * may differ in N (<= 32), lattice, couplings and fields.  Every Lanczos step
+ * of the whole batch is one fixed launch sequence over a task list of
+ * (model, row range) pairs; steps are replayed from a captured hipGraph and
+ * the host retires converged models between replays.  Convergence: the
+ * lowest Ritz pair of the tridiagonal matrix has |beta_m s_m| / |theta| < tol
+ * (the EPS_CONV_REL test of the Krylov-Schur solver).
+ *   maxit            Lanczos steps per model (<= 0: 300)
+ *   tol              (<= 0: 1e-9)
+ *   seed             start vectors (0: 1)
+ *   lanes_per_state  0: autotune the launch shape (rows per task, lanes per
+ *                    basis state, blocks per CU) by timing it on the batch,
+ *                    cached per batch shape; 1, 2, 4: fixed lanes, default tile
+ *   keep_vectors     1: a second, identical pass builds the ground-state
+ *                    vectors (one more D-vector per model) */
+typedef struct {
+  int maxit;
+  double tol;
+  uint64_t seed;
+  int lanes_per_state;
+  int keep_vectors;
+} lsqed_batch_opts;
+
+typedef struct lsqed_batch lsqed_batch;
+
+/* Device bytes lsqed_batch_create allocates for the vectors (3 or 4 doubles
+ * per basis state of every model); -1 on invalid input. */
+int64_t lsqed_batch_workspace_bytes(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o);
+/* The models are referenced, not copied: set_fields / set_couplings /
+ * set_build on them between solves take effect at the next solve. */
+int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o, lsqed_batch** out);
+/* e0: n doubles (host), the lowest eigenvalue of each model. */
+int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream);
+/* Lanczos steps, relative residual and convergence flag of model k. */
+int lsqed_batch_info(const lsqed_batch* b, int k, int* steps, double* resid, int* converged);
+/* Launch shape in use (the autotuner's choice). */
+int lsqed_batch_schedule(const lsqed_batch* b, int* rows_per_task, int* lanes_per_state, int* blocks_per_cu);
+/* Unit-norm ground state of model k into psi (D_k doubles, device). */
+int lsqed_batch_ground_state(lsqed_batch* b, int k, double* psi, lsqed_stream_t stream);
+/* Hellmann-Feynman derivatives of every model's E0 from its ground state,
+ * out[k*LSQED_BATCH_OBS_STRIDE + j] (host):
+ *   j < N : <S^z_j>                                    (= dE0/dh_j)
+ *   32    : <psi|psi> before normalisation
+ *   33/35 : sum over nn / nnn bonds of <S^z_i S^z_j>    (= dE0/dD1, dE0/dD2)
+ *   34/36 : sum over nn / nnn bonds of <(S^+_i S^-_j + S^-_i S^+_j)/2>
+ *                                                       (= dE0/dJ1, dE0/dJ2)
+ * The correlators are SzSz/SpSm::ExpectVal (src/spinOperators/
+ * correlation.cpp:86-224) summed over the Hamiltonian's bonds. */
+#define LSQED_BATCH_OBS_STRIDE 40
+int lsqed_batch_observables(lsqed_batch* b, double* out, lsqed_stream_t stream);
+int lsqed_batch_destroy(lsqed_batch* b);
 
 /* ---------------------------------------------------------------- misc ---- */
 /* Tools::RandomDisorder (src/tools/random_disorder.cpp:33-59): the

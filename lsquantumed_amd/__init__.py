@@ -9,5 +9,7 @@ from .model import AF, Basis, Environment, Hamiltonian, Striped, chain1D, honeyc
 from .phys import Phys, SiSj, SmSp, SpSm, SzSz, correlations  # noqa: F401
 from .solver import EPS, PETSC_DECIDE, PETSC_DEFAULT, Solver, lanczos_ground, lanczos_step  # noqa: F401
 from .tools import RandomDisorder, fields_from_seed  # noqa: F401
+from .batch import BatchSolver, disorder_chains  # noqa: F401
+from .autograd import ground_energy  # noqa: F401
 
 __version__ = "0.1.0"

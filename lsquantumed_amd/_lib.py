@@ -58,6 +58,7 @@ SIGNATURES = {
     "lsqed_hv": (_i32, [_vp, _vp, _vp, _vp]),
     "lsqed_ham_set_hv_kernel": (_i32, [_vp, _i32]),
     "lsqed_ham_get_hv_kernel": (_i32, [_vp]),
+    "lsqed_ham_set_couplings": (_i32, [_vp, _dbl, _dbl, _dbl, _dbl]),
     "lsqed_diag": (_i32, [_vp, _vp, _vp]),
     "lsqed_export_cols": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _vp]),
     "lsqed_unrank": (_i32, [_vp, _i64, _i64, _vp, _vp]),
@@ -75,7 +76,10 @@ SIGNATURES = {
     "lsqed_obs_dsf": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lsqed_batch_create": (_i32, [_i32, _P(_vp), _P(BatchOpts), _P(_vp)]),
     "lsqed_batch_solve": (_i32, [_vp, _vp, _vp]),
+    "lsqed_batch_info": (_i32, [_vp, _i32, _P(_i32), _P(_dbl), _P(_i32)]),
+    "lsqed_batch_schedule": (_i32, [_vp, _P(_i32), _P(_i32), _P(_i32)]),
     "lsqed_batch_ground_state": (_i32, [_vp, _i32, _vp, _vp]),
+    "lsqed_batch_observables": (_i32, [_vp, _vp, _vp]),
     "lsqed_batch_destroy": (_i32, [_vp]),
     "lsqed_batch_workspace_bytes": (_i64, [_i32, _P(_vp), _P(BatchOpts)]),
     "lsqed_rd_seeds": (_i32, [_i32, _vp]),

@@ -486,6 +486,18 @@ int lsqed_ham_set_fields(lsqed_ham* h, const double* hi) {
   return LSQED_OK;
 }
 
+int lsqed_ham_set_couplings(lsqed_ham* h, double J1, double D1, double J2, double D2) {
+  if (!h) return fail(LSQED_ERR_ARG, "null handle");
+  if (!std::isfinite(J1) || !std::isfinite(D1) || !std::isfinite(J2) || !std::isfinite(D2))
+    return fail(LSQED_ERR_ARG, "couplings must be finite");
+  h->J[0] = J1;
+  h->Dz[0] = D1;
+  h->J[1] = J2;
+  h->Dz[1] = D2;
+  h->jit = nullptr;  // the specialised kernel bakes in the couplings
+  return LSQED_OK;
+}
+
 int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream) {
   if (!h || !x || !y) return fail(LSQED_ERR_ARG, "null argument");
   if (x == y) return fail(LSQED_ERR_ARG, "x and y must not alias");

@@ -210,6 +210,13 @@ class Hamiltonian:
         check(lib().lsqed_ham_set_build(self.hamilt, self._flags))
         return 0
 
+    def set_couplings(self, J1: float, D1: float, J2: float, D2: float) -> None:
+        """New couplings for this model, as if constructed with them
+        (tmpl:69-91); lattice, sector and build calls are kept."""
+        J1, D1, J2, D2 = float(J1), float(D1), float(J2), float(D2)
+        check(lib().lsqed_ham_set_couplings(self.hamilt, J1, D1, J2, D2))
+        self.J1, self.Delta1, self.J2, self.Delta2 = J1, D1, J2, D2
+
     # --- accessors (include/hamiltonian.hpp:257-299) -----------------------
     def hamilt_size(self) -> int:
         return self.size

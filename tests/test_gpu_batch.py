@@ -1,0 +1,241 @@
+"""GPU parity of the batched disorder solver and the differentiable E0.
+
+Batched ground energies against the oracle's dense spectra (small chains)
+and against the plain one-at-a-time Krylov-Schur solve (the loop of
+examples/main_disorder.cpp:82-120) on samples of the N = 18/20/22 disorder
+workload; ground states and their Hellmann-Feynman observables against the
+oracle's eigenvectors and correlators; autograd gradients against central
+finite differences (step 1e-5) at 1e-6 relative."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FLAGS_DIS = 1 | 2 | 4  # build_diag + build_off_diag + build_disorder
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as T
+    assert T.cuda.is_available()
+    return T
+
+
+def oracle_ground(O, N, J1, J2, hi, vectors=False):
+    m = O.Model(O.CHAIN, N, N // 2, J1, J1, J2, J2)
+    A = m.csr(hi, FLAGS_DIS)
+    w, v = O.eigs(A, 1)
+    return (w[0], v[:, 0]) if vectors else w[0]
+
+
+def plain_e0(h, tol=1e-12):
+    from lsquantumed_amd import Solver
+    eps = Solver.SolverInit(h, 1, tol=tol)
+    nconv = Solver.solve(None, eps)
+    assert nconv >= 1
+    lam, err = Solver.solution(eps, 0)
+    Solver.SolverClean(eps)
+    return lam
+
+
+def test_batch_small_chains_vs_oracle(O, torch):
+    """Mixed N = 8/10/12 chains with J2 and random fields: every E0 equals the
+    lowest eigenvalue of the oracle's matrix (dense) to 1e-11 relative."""
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    sizes, counts = [8, 10, 12], [3, 3, 4]
+    hams = disorder_chains(sizes, counts, J1=1.0, J2=0.2, W=2.0, seed=1)
+    say("batch small", sizes, counts)
+    b = BatchSolver(hams, tol=1e-12)
+    e0 = b.solve()
+    for k, h in enumerate(hams):
+        ref = oracle_ground(O, h.basis.nspins, 1.0, 0.2, h.hi)
+        assert abs(e0[k] - ref) <= 1e-11 * abs(ref), (k, e0[k], ref)
+        steps, res, conv = b.info(k)
+        assert conv and res < 1e-12 and steps >= 2
+    sched = b.schedule()
+    assert sched["lanes_per_state"] in (1, 2, 4) and sched["rows_per_task"] > 0
+
+
+def test_batch_disorder_sample_vs_plain(O, torch):
+    """Sample of the N = 18/20/22 workload (J1 = 1, J2 = 0.2, h ~ U[-2, 2] from
+    mt19937_64(1 + k)): batched E0 = plain Krylov-Schur E0 to 1e-9 relative,
+    and one N = 18 value against the oracle's sparse eigensolve."""
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    sizes, counts = [18, 20, 22], [4, 4, 8]
+    hams = disorder_chains(sizes, counts, J1=1.0, J2=0.2, W=2.0, seed=1)
+    say("batch sample", sizes, counts)
+    b = BatchSolver(hams, tol=1e-9)
+    e0 = b.solve()
+    for k, h in enumerate(hams):
+        ref = plain_e0(h)
+        assert abs(e0[k] - ref) <= 1e-9 * abs(ref), (k, e0[k], ref)
+        assert b.info(k)[2]
+    ref0 = oracle_ground(O, 18, 1.0, 0.2, hams[0].hi)
+    assert abs(e0[0] - ref0) <= 1e-9 * abs(ref0)
+
+
+def test_batch_lanes_per_state_agree(torch):
+    """Fixed lanes-per-state schedules (1, 2, 4) and the autotuned one give
+    the same E0 (the Lanczos recurrence is the same up to rounding)."""
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    hams = disorder_chains([12, 14], [3, 3], J1=1.0, J2=0.2, W=2.0, seed=7)
+    vals = []
+    for lps in (0, 1, 2, 4):
+        say("lanes", lps)
+        b = BatchSolver(hams, tol=1e-12, lanes_per_state=lps)
+        vals.append(b.solve())
+        if lps:
+            assert b.schedule()["lanes_per_state"] == lps
+        b.destroy()
+    for v in vals[1:]:
+        np.testing.assert_allclose(v, vals[0], rtol=1e-12, atol=0)
+
+
+def test_batch_resolve_after_new_fields(O, torch):
+    """Fields set on the models between solves take effect at the next solve
+    (the reference rebuilds the disorder per realisation, main_disorder.cpp:83-84)."""
+    from lsquantumed_amd import fields_from_seed
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    hams = disorder_chains([10, 12], [2, 2], J1=1.0, J2=0.2, W=2.0, seed=3)
+    b = BatchSolver(hams, tol=1e-12)
+    b.solve()
+    for k, h in enumerate(hams):
+        h.build_disorder(None, fields_from_seed(100 + k, h.basis.nspins, -1.0, 1.0))
+    e0 = b.solve()
+    for k, h in enumerate(hams):
+        ref = oracle_ground(O, h.basis.nspins, 1.0, 0.2, h.hi)
+        assert abs(e0[k] - ref) <= 1e-11 * abs(ref)
+
+
+def test_batch_vectors_and_observables(O, torch):
+    """Ground states of the second pass: unit norm, eigenvectors of H
+    (||H psi - E0 psi|| <= 1e-9), equal to the oracle's up to sign; the
+    observables kernel gives <S^z_i> and the bond sums of <S^z S^z> and
+    <(S^+S^- + S^-S^+)/2> of the oracle's correlators (SzSz/SpSm::ExpectVal)
+    to 1e-9, and E0 = J1 dE/dJ1 + J2 dE/dJ2 + sum_i h_i dE/dh_i (H is linear
+    in its parameters)."""
+    from lsquantumed_amd.batch import (OBS_FLIP_NN, OBS_FLIP_NNN, OBS_NORM, OBS_ZZ_NN, OBS_ZZ_NNN, BatchSolver,
+                                       disorder_chains)
+    J1, J2 = 1.0, 0.3
+    hams = disorder_chains([10, 12], [2, 3], J1=J1, J2=J2, W=1.0, seed=11)
+    b = BatchSolver(hams, tol=1e-12, keep_vectors=True)
+    e0 = b.solve()
+    obs = b.observables()
+    for k, h in enumerate(hams):
+        N = h.basis.nspins
+        nup = N // 2
+        say("vectors", k, N)
+        psi = b.ground_state(k)
+        assert abs(torch.linalg.vector_norm(psi).item() - 1.0) < 1e-12
+        r = h.matvec(psi) - e0[k] * psi
+        assert torch.linalg.vector_norm(r).item() <= 1e-9 * abs(e0[k])
+        ref_e, ref_v = oracle_ground(O, N, J1, J2, h.hi, vectors=True)
+        p = psi.cpu().numpy()
+        assert abs(abs(p @ ref_v) - 1.0) < 1e-10
+        # <S^z_i>
+        states = O.int_basis(N, nup)
+        w = ref_v ** 2
+        sz = np.array([np.sum(w * np.where((states >> i) & 1, 0.5, -0.5)) for i in range(N)])
+        np.testing.assert_allclose(obs[k, :N], sz, rtol=0, atol=1e-9)
+        assert abs(obs[k, OBS_NORM] - 1.0) < 1e-12
+        # bond sums from the oracle's correlators
+        for kind, (zz_i, fl_i) in enumerate([(OBS_ZZ_NN, OBS_FLIP_NN), (OBS_ZZ_NNN, OBS_FLIP_NNN)]):
+            bi, bj = O.bonds(O.CHAIN, N, kind=kind)
+            zz = sum(O.szsz(N, nup, ref_v, int(i), int(j)) for i, j in zip(bi, bj))
+            fl = sum(0.5 * (O.spsm(N, nup, ref_v, int(i), int(j)) + O.smsp(N, nup, ref_v, int(i), int(j)))
+                     for i, j in zip(bi, bj))
+            assert abs(obs[k, zz_i] - zz) < 1e-9, (k, kind, obs[k, zz_i], zz)
+            assert abs(obs[k, fl_i] - fl) < 1e-9, (k, kind, obs[k, fl_i], fl)
+        euler = (J1 * (obs[k, OBS_ZZ_NN] + obs[k, OBS_FLIP_NN]) + J2 * (obs[k, OBS_ZZ_NNN] + obs[k, OBS_FLIP_NNN])
+                 + float(np.dot(h.hi, obs[k, :N])))
+        assert abs(euler - e0[k]) <= 1e-10 * abs(e0[k])
+
+
+def test_batch_errors(torch):
+    import lsquantumed_amd as Q
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    hams = disorder_chains([8], [2], J1=1.0, J2=0.0, W=1.0)
+    with pytest.raises(Q.LsqedError):
+        BatchSolver(hams, lanes_per_state=3)
+    with pytest.raises(ValueError):
+        BatchSolver([])
+    b = BatchSolver(hams)
+    with pytest.raises(Q.LsqedError):
+        b.info(0)  # not solved
+    b.solve()
+    with pytest.raises(Q.LsqedError):
+        b.ground_state(0)  # no vectors kept
+    with pytest.raises(Q.LsqedError):
+        b.observables()
+    with pytest.raises(Q.LsqedError):
+        b.info(5)
+    env = Q.Environment(34)
+    big = Q.Hamiltonian(env, Q.Basis(env, 0), Q.chain1D(env), 1.0, 1.0, 0.0, 0.0)
+    with pytest.raises(Q.LsqedError):
+        BatchSolver([big])
+
+
+def _fd_check(torch, B, N, J1v, J2v, seed, fd_rows):
+    from lsquantumed_amd.autograd import ground_energy
+    from lsquantumed_amd.tools import fields_from_seed
+    h0 = np.stack([fields_from_seed(seed + k, N, -1.0, 1.0) for k in range(B)])
+    h = torch.tensor(h0, dtype=torch.float64, requires_grad=True)
+    J1 = torch.tensor(J1v, dtype=torch.float64, requires_grad=True)
+    J2 = torch.tensor(J2v, dtype=torch.float64, requires_grad=True)
+    say("autograd forward", B, N)
+    E = ground_energy(h, J1, J2)
+    E.sum().backward()
+    gh, gj1, gj2 = h.grad.numpy(), J1.grad.item(), J2.grad.item()
+    d = 1e-5
+    # dE_b/dh_bi for the chosen realisations: one batched solve of all +/- perturbations
+    rows = []
+    for b in fd_rows:
+        for i in range(N):
+            for s in (+1, -1):
+                r = h0[b].copy()
+                r[i] += s * d
+                rows.append(r)
+    say("fd fields", len(rows))
+    with torch.no_grad():
+        Ep = ground_energy(torch.tensor(np.array(rows)), J1v, J2v).numpy().reshape(len(fd_rows), N, 2)
+    fd_h = (Ep[:, :, 0] - Ep[:, :, 1]) / (2 * d)
+    for n, b in enumerate(fd_rows):
+        err = np.linalg.norm(fd_h[n] - gh[b]) / np.linalg.norm(gh[b])
+        say("dE/dh", b, err)
+        assert err <= 1e-6, (b, err)
+    with torch.no_grad():
+        hh = torch.tensor(h0)
+        fj1 = (ground_energy(hh, J1v + d, J2v).sum() - ground_energy(hh, J1v - d, J2v).sum()).item() / (2 * d)
+        fj2 = (ground_energy(hh, J1v, J2v + d).sum() - ground_energy(hh, J1v, J2v - d).sum()).item() / (2 * d)
+    say("dE/dJ", gj1, fj1, gj2, fj2)
+    assert abs(fj1 - gj1) <= 1e-6 * abs(gj1)
+    assert abs(fj2 - gj2) <= 1e-6 * abs(gj2)
+    return E.detach().numpy(), h0
+
+
+def test_autograd_small_fd(torch):
+    """E0(h, J1, J2) gradients on 4 chains of 12 spins against central
+    differences (step 1e-5): norm-wise relative error <= 1e-6."""
+    _fd_check(torch, B=4, N=12, J1v=1.0, J2v=0.3, seed=5, fd_rows=[0, 1, 2, 3])
+
+
+def test_autograd_chain20_batch64(O, torch):
+    """The differentiable-E0 workload: 64 chains of 20 spins, J1 = 1,
+    J2 = 0.3, h ~ U[-1, 1] (mt19937_64(1 + k)); gradients of the batch sum
+    against central differences at 1e-6 relative (dE/dh for two
+    realisations, dE/dJ1 and dE/dJ2 over the whole batch); E0 of two
+    realisations against the plain Krylov-Schur solve at 1e-9."""
+    from lsquantumed_amd.batch import disorder_chains
+    E, h0 = _fd_check(torch, B=64, N=20, J1v=1.0, J2v=0.3, seed=1, fd_rows=[0, 37])
+    hams = disorder_chains([20], [2], J1=1.0, J2=0.3, W=1.0, seed=1)
+    for k in range(2):
+        np.testing.assert_allclose(hams[k].hi, h0[k], rtol=0, atol=0)
+        ref = plain_e0(hams[k])
+        assert abs(E[k] - ref) <= 1e-9 * abs(ref)
