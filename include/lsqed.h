@@ -210,13 +210,17 @@ int lsqed_obs_dsf(lsqed_ham* h0, lsqed_ham* h1, const double* psi0, const double
  *                    basis state, blocks per CU) by timing it on the batch,
  *                    cached per batch shape; 1, 2, 4: fixed lanes, default tile
  *   keep_vectors     1: a second, identical pass builds the ground-state
- *                    vectors (one more D-vector per model) */
+ *                    vectors (one more D-vector per model)
+ *   hv_kernel        0: autotune chooses; 1: chunk-table kernel over the task
+ *                    list; 2: row split specialised per lattice group (hipRTC,
+ *                    couplings and fields per model at run time) */
 typedef struct {
   int maxit;
   double tol;
   uint64_t seed;
   int lanes_per_state;
   int keep_vectors;
+  int hv_kernel;
 } lsqed_batch_opts;
 
 typedef struct lsqed_batch lsqed_batch;
@@ -231,8 +235,12 @@ int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o,
 int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream);
 /* Lanczos steps, relative residual and convergence flag of model k. */
 int lsqed_batch_info(const lsqed_batch* b, int k, int* steps, double* resid, int* converged);
-/* Launch shape in use (the autotuner's choice). */
-int lsqed_batch_schedule(const lsqed_batch* b, int* rows_per_task, int* lanes_per_state, int* blocks_per_cu);
+/* Launch shape in use (the autotuner's choice): H.v kernel (1 chunk table,
+ * 2 specialised row split), rows per task, lanes per basis state, blocks per
+ * CU, and whether replays run from a captured hipGraph (1) or are enqueued
+ * launch by launch (0).  Any pointer may be NULL. */
+int lsqed_batch_schedule(const lsqed_batch* b, int* hv_kernel, int* rows_per_task, int* lanes_per_state,
+                         int* blocks_per_cu, int* graph);
 /* Unit-norm ground state of model k into psi (D_k doubles, device). */
 int lsqed_batch_ground_state(lsqed_batch* b, int k, double* psi, lsqed_stream_t stream);
 /* Hellmann-Feynman derivatives of every model's E0 from its ground state,

@@ -38,7 +38,7 @@ class EpsOpts(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("maxit", C.c_int), ("tol", C.c_double), ("seed", C.c_uint64), ("lanes_per_state", C.c_int),
-                ("keep_vectors", C.c_int)]
+                ("keep_vectors", C.c_int), ("hv_kernel", C.c_int)]
 
 
 _vp, _i32, _i64, _u64, _dbl = C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_double
@@ -77,7 +77,7 @@ SIGNATURES = {
     "lsqed_batch_create": (_i32, [_i32, _P(_vp), _P(BatchOpts), _P(_vp)]),
     "lsqed_batch_solve": (_i32, [_vp, _vp, _vp]),
     "lsqed_batch_info": (_i32, [_vp, _i32, _P(_i32), _P(_dbl), _P(_i32)]),
-    "lsqed_batch_schedule": (_i32, [_vp, _P(_i32), _P(_i32), _P(_i32)]),
+    "lsqed_batch_schedule": (_i32, [_vp, _P(_i32), _P(_i32), _P(_i32), _P(_i32), _P(_i32)]),
     "lsqed_batch_ground_state": (_i32, [_vp, _i32, _vp, _vp]),
     "lsqed_batch_observables": (_i32, [_vp, _vp, _vp]),
     "lsqed_batch_destroy": (_i32, [_vp]),

@@ -29,14 +29,17 @@ OBS_NORM, OBS_ZZ_NN, OBS_FLIP_NN, OBS_ZZ_NNN, OBS_FLIP_NNN = 32, 33, 34, 35, 36
 class BatchSolver:
     """Handle of one batch (lsqed_batch_* in include/lsqed.h)."""
 
+    HV_AUTO, HV_CHUNK_TABLE, HV_SPECIALISED = 0, 1, 2
+
     def __init__(self, hams, maxit: int = 300, tol: float = 1e-9, seed: int = 1, lanes_per_state: int = 0,
-                 keep_vectors: bool = False):
+                 keep_vectors: bool = False, hv_kernel: int = 0):
         self.hams = list(hams)
         if not self.hams:
             raise ValueError("empty batch")
         n = len(self.hams)
         self._arr = (C.c_void_p * n)(*[h.hamilt.value for h in self.hams])
-        o = _lib.BatchOpts(int(maxit), float(tol), int(seed), int(lanes_per_state), int(bool(keep_vectors)))
+        o = _lib.BatchOpts(int(maxit), float(tol), int(seed), int(lanes_per_state), int(bool(keep_vectors)),
+                           int(hv_kernel))
         hdl = C.c_void_p()
         check(lib().lsqed_batch_create(n, self._arr, C.byref(o), C.byref(hdl)))
         self.handle = hdl
@@ -46,7 +49,7 @@ class BatchSolver:
     @staticmethod
     def workspace_bytes(hams, keep_vectors: bool = False) -> int:
         arr = (C.c_void_p * len(hams))(*[h.hamilt.value for h in hams])
-        o = _lib.BatchOpts(0, 0.0, 0, 0, int(bool(keep_vectors)))
+        o = _lib.BatchOpts(0, 0.0, 0, 0, int(bool(keep_vectors)), 0)
         return int(lib().lsqed_batch_workspace_bytes(len(hams), arr, C.byref(o)))
 
     def destroy(self):
@@ -74,9 +77,11 @@ class BatchSolver:
 
     def schedule(self) -> dict:
         """Launch shape in use (the autotuner's choice)."""
-        r, l, g = C.c_int(), C.c_int(), C.c_int()
-        check(lib().lsqed_batch_schedule(self.handle, C.byref(r), C.byref(l), C.byref(g)))
-        return {"rows_per_task": r.value, "lanes_per_state": l.value, "blocks_per_cu": g.value}
+        k, r, l, g, gr = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().lsqed_batch_schedule(self.handle, C.byref(k), C.byref(r), C.byref(l), C.byref(g), C.byref(gr)))
+        return {"hv_kernel": {1: "chunk table", 2: "specialised row split"}.get(k.value, "?"),
+                "rows_per_task": r.value, "lanes_per_state": l.value, "blocks_per_cu": g.value,
+                "graph": bool(gr.value)}
 
     def ground_state(self, k: int, out=None, stream=None):
         import torch

@@ -48,6 +48,7 @@ struct BatchProb {
   RowParams p;     // sector tables, bonds, fields, couplings; p.D = this problem's dimension
   int64_t off;     // offset of this problem's entries in the batched vectors
   int task0, task1;  // its task range
+  int hv0, hv1;      // its range of H.v partial sums (the task range, or the specialised kernel's)
 };
 
 // Task t covers rows [row0, row0 + nrows) of problem prob.
@@ -178,14 +179,16 @@ __global__ __launch_bounds__(kBThreads) void k_batch_hv(const BatchProb* __restr
   }
 }
 
-// out[k] = sum of the task partials of problem k (fixed order: deterministic).
+// out[k] = sum of the partials of problem k, over its task range (hv = 0)
+// or its H.v partial range (hv = 1), in a fixed order (deterministic).
 __global__ void k_batch_reduce(const BatchProb* __restrict__ probs, int nprob, const double* __restrict__ part,
-                               double* __restrict__ out) {
+                               double* __restrict__ out, int hv) {
   __shared__ double red[4];
   const int k = blockIdx.x;
   if (k >= nprob) return;
+  const int t0 = hv ? probs[k].hv0 : probs[k].task0, t1 = hv ? probs[k].hv1 : probs[k].task1;
   double acc = 0.0;
-  for (int t = probs[k].task0 + threadIdx.x; t < probs[k].task1; t += blockDim.x) acc += part[t];
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) acc += part[t];
   const double tot = block_sum(acc, red);
   if (threadIdx.x == 0) out[k] = tot;
 }
@@ -435,9 +438,21 @@ namespace {
 enum { S_QY = 0, S_NN, S_SQ, S_SP, S_BETA, S_COEF, S_W = S_COEF + 3, S_COUNT = S_W + kSteps };
 
 struct Shape {
+  int kernel = 0;   // H.v: 0 chunk-table kernel over the task list, 1 specialised row split per lattice group
   int rows = 4096;  // rows per task
-  int lps = 1;      // lanes per basis state
+  int lps = 1;      // lanes per basis state (kernel 0)
   int bpc = 4;      // blocks per CU
+  bool operator!=(const Shape& o) const { return kernel != o.kernel || rows != o.rows || lps != o.lps || bpc != o.bpc; }
+};
+
+// Problems that share a bond structure and build mode (one specialised kernel).
+struct Group {
+  void* module = nullptr;
+  lsqed_ham* h0 = nullptr;  // its split tables serve the whole group
+  std::vector<int> idx;     // batch indices
+  int ntasks = 0;           // row-split tasks per problem
+  int part0 = 0;            // first H.v partial of the group
+  int gofs = 0;             // offset of its indices in d_gidx
 };
 
 std::mutex g_tune_mu;
@@ -452,7 +467,6 @@ struct lsqed_batch {
   int64_t total = 0;
   std::vector<int64_t> off, dims;
   Shape shape;
-  bool tuned = false;
   int grid = 0, ntasks = 0;
   size_t smem = 0;
   int cus = 256;
@@ -466,6 +480,17 @@ struct lsqed_batch {
   double* d_s = nullptr;     // S_COUNT * nprob scalars
   double* d_hist = nullptr;  // [2][kSteps][nprob]: alpha then beta of each step of a replay
   double* d_obs = nullptr;   // nprob * kObsStride
+  // specialised H.v per lattice group
+  std::vector<Group> groups;
+  bool jit_ok = false;
+  int64_t* d_poff = nullptr;
+  double* d_pcoup = nullptr;   // 4 per problem
+  double* d_pfields = nullptr; // 32 per problem
+  int* d_gidx = nullptr;
+  double* d_hvpart = nullptr;
+  size_t hvpart_cap = 0;
+  std::vector<int> prob_hv0;
+  bool use_graph = true;
   // pinned host
   double* h_hist = nullptr;  // [2 buffers][2][kSteps][nprob]
   int* h_active = nullptr;   // ring of activity vectors, [ring][nprob]
@@ -488,10 +513,86 @@ double* hist_alpha(lsqed_batch* b, int j) { return b->d_hist + (size_t)j * b->np
 double* hist_beta(lsqed_batch* b, int j) { return b->d_hist + (size_t)(kSteps + j) * b->nprob; }
 
 std::string shape_key(const lsqed_batch* b) {
-  std::string key;
+  std::string key = "k" + std::to_string(b->opts.hv_kernel) + "l" + std::to_string(b->opts.lanes_per_state) +
+                    (b->jit_ok ? "j;" : ";");
   for (const lsqed_ham* h : b->hams)
-    key += std::to_string(h->N) + "/" + std::to_string(h->bi[0].size()) + "/" + std::to_string(h->bi[1].size()) + ";";
+    key += std::to_string(h->N) + "/" + std::to_string(h->bi[0].size()) + "/" + std::to_string(h->bi[1].size()) +
+           "/" + std::to_string(h->build) + ";";
   return key;
+}
+
+// Lattice groups of the specialised H.v: problems whose generated kernel
+// source (bond structure, build mode) is the same share one module.  Rebuilt
+// at every solve, since build flags decide the kernel.  When some problem
+// cannot use it (N > 32, too many high bonds, hipRTC unavailable) the batch
+// keeps the chunk-table kernel only.
+int setup_groups(lsqed_batch* b) {
+  const int np = b->nprob;
+  b->groups.clear();
+  b->jit_ok = false;
+  b->prob_hv0.assign(np, 0);
+  if (!jit_enabled() || b->opts.hv_kernel == 1) return LSQED_OK;
+  std::map<std::string, int> gid;
+  std::vector<int> pl(np, 0);
+  for (int k = 0; k < np; ++k) {
+    lsqed_ham* h = b->hams[k];
+    if (!split_supported(h) || !split_jit_fits(h)) return LSQED_OK;
+    void* mod = nullptr;
+    std::string key;
+    if (batch_jit_prepare(h, &mod, &key) != LSQED_OK) return LSQED_OK;
+    auto it = gid.find(key);
+    if (it == gid.end()) {
+      it = gid.emplace(key, (int)b->groups.size()).first;
+      Group g;
+      g.module = mod;
+      g.h0 = h;
+      b->groups.push_back(g);
+    }
+    Group& g = b->groups[it->second];
+    pl[k] = (int)g.idx.size();
+    g.idx.push_back(k);
+  }
+  int part = 0, gofs = 0;
+  std::vector<int> gidx;
+  for (Group& g : b->groups) {
+    const int rc = split_task_count(g.h0, &g.ntasks);
+    if (rc) return rc;
+    g.part0 = part;
+    g.gofs = gofs;
+    for (int k : g.idx) b->prob_hv0[k] = part + pl[k] * g.ntasks;
+    part += g.ntasks * (int)g.idx.size();
+    gofs += (int)g.idx.size();
+    gidx.insert(gidx.end(), g.idx.begin(), g.idx.end());
+  }
+  if ((size_t)part > b->hvpart_cap) {
+    (void)hipFree(b->d_hvpart);
+    b->d_hvpart = nullptr;
+    LSQED_HIP(hipMalloc((void**)&b->d_hvpart, (size_t)part * sizeof(double)));
+    b->hvpart_cap = part;
+  }
+  std::vector<double> coup(4 * (size_t)np), fields(32 * (size_t)np, 0.0);
+  for (int k = 0; k < np; ++k) {
+    const lsqed_ham* h = b->hams[k];
+    const RowParams rp = make_row_params(h);
+    coup[4 * k + 0] = rp.jh[0];
+    coup[4 * k + 1] = rp.jh[1];
+    coup[4 * k + 2] = rp.dq[0];
+    coup[4 * k + 3] = rp.dq[1];
+    if (rp.diag_mode == 1)
+      for (int i = 0; i < h->N; ++i) fields[32 * k + i] = h->fields[i];
+  }
+  LSQED_HIP(hipMemcpy(b->d_gidx, gidx.data(), np * sizeof(int), hipMemcpyHostToDevice));
+  LSQED_HIP(hipMemcpy(b->d_poff, b->off.data(), np * sizeof(int64_t), hipMemcpyHostToDevice));
+  LSQED_HIP(hipMemcpy(b->d_pcoup, coup.data(), coup.size() * sizeof(double), hipMemcpyHostToDevice));
+  LSQED_HIP(hipMemcpy(b->d_pfields, fields.data(), fields.size() * sizeof(double), hipMemcpyHostToDevice));
+  b->jit_ok = true;
+  return LSQED_OK;
+}
+
+int grid_for(const lsqed_batch* b, int64_t work_items) {
+  int64_t g = (int64_t)b->cus * b->shape.bpc;
+  g = std::min<int64_t>(g, std::max<int64_t>(work_items, 1));
+  return (int)std::max<int64_t>(8, ((g + 7) / 8) * 8);
 }
 
 // Upload problem descriptors (fields pointers and build modes may have
@@ -500,6 +601,7 @@ int upload_problems(lsqed_batch* b) {
   std::vector<BTask> tasks;
   std::vector<BatchProb> probs(b->nprob);
   const int rows = b->shape.rows;
+  const bool spec = b->shape.kernel == 1;
   for (int k = 0; k < b->nprob; ++k) {
     lsqed_ham* h = b->hams[k];
     BatchProb& bp = probs[k];
@@ -509,6 +611,16 @@ int upload_problems(lsqed_batch* b) {
     bp.task0 = (int)tasks.size();
     for (int64_t r = 0; r < h->D; r += rows) tasks.push_back(BTask{k, (int)r, (int)std::min<int64_t>(rows, h->D - r)});
     bp.task1 = (int)tasks.size();
+    if (spec) {
+      const Group* g = nullptr;
+      for (const Group& gg : b->groups)
+        if (std::find(gg.idx.begin(), gg.idx.end(), k) != gg.idx.end()) g = &gg;
+      bp.hv0 = b->prob_hv0[k];
+      bp.hv1 = bp.hv0 + (g ? g->ntasks : 0);
+    } else {
+      bp.hv0 = bp.task0;
+      bp.hv1 = bp.task1;
+    }
   }
   if ((int)tasks.size() != b->ntasks) {
     (void)hipFree(b->d_tasks);
@@ -525,13 +637,25 @@ int upload_problems(lsqed_batch* b) {
   }
   LSQED_HIP(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice));
   LSQED_HIP(hipMemcpy(b->d_probs, probs.data(), probs.size() * sizeof(BatchProb), hipMemcpyHostToDevice));
-  int64_t g = (int64_t)b->cus * b->shape.bpc;
-  g = std::min<int64_t>(g, b->ntasks);
-  b->grid = (int)std::max<int64_t>(8, ((g + 7) / 8) * 8);
+  b->grid = grid_for(b, b->ntasks);
   return LSQED_OK;
 }
 
-void launch_hv_batch(lsqed_batch* b, const double* q, double* y, hipStream_t s) {
+// y = H q for every active problem; the per-problem partials of q.y go to
+// the array returned by hv_partials().
+int launch_hv_batch(lsqed_batch* b, const double* q, double* y, hipStream_t s) {
+  if (b->shape.kernel == 1) {
+    int64_t work = 0;
+    for (const Group& g : b->groups) work += (int64_t)g.ntasks * (int64_t)g.idx.size();
+    const int grid = grid_for(b, work);
+    for (const Group& g : b->groups) {
+      BatchJitLaunch L{q,           y,           b->d_hvpart + g.part0, b->d_poff, b->d_pcoup, b->d_pfields,
+                       b->d_active, b->d_gidx + g.gofs, (int)g.idx.size()};
+      const int rc = launch_hv_batch_jit(g.module, g.h0, L, grid, s);
+      if (rc) return rc;
+    }
+    return LSQED_OK;
+  }
   switch (b->shape.lps) {
     case 2:
       hipLaunchKernelGGL(k_batch_hv<2>, dim3(b->grid), dim3(kBThreads), b->smem, s, b->d_probs, b->d_tasks,
@@ -545,25 +669,30 @@ void launch_hv_batch(lsqed_batch* b, const double* q, double* y, hipStream_t s) 
       hipLaunchKernelGGL(k_batch_hv<1>, dim3(b->grid), dim3(kBThreads), b->smem, s, b->d_probs, b->d_tasks,
                          b->ntasks, b->d_active, q, y, b->d_part);
   }
+  return LSQED_OK;
 }
+
+double* hv_partials(lsqed_batch* b) { return b->shape.kernel == 1 ? b->d_hvpart : b->d_part; }
 
 // One Lanczos step on the current vector q (scale sq) and previous p; the
 // new vector overwrites p.  Scalars of the step go to history slot j.
-void enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, hipStream_t s) {
+int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, hipStream_t s) {
   const int np = b->nprob;
   const int sb = (np + 255) / 256;
-  launch_hv_batch(b, q, b->d_y, s);
+  const int rc = launch_hv_batch(b, q, b->d_y, s);
+  if (rc) return rc;
   if (accumulate)
     hipLaunchKernelGGL(k_batch_accum, dim3(b->grid), dim3(kBThreads), 0, s, b->d_probs, b->d_tasks, b->ntasks,
                        b->d_active, sc(b, S_W + j), sc(b, S_SQ), q, b->d_psi);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_QY));
+  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, hv_partials(b), sc(b, S_QY), 1);
   hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_QY), sc(b, S_SQ),
                      sc(b, S_SP), sc(b, S_BETA), hist_alpha(b, j), sc(b, S_COEF));
   hipLaunchKernelGGL(k_batch_update, dim3(b->grid), dim3(kBThreads), 0, s, b->d_probs, b->d_tasks, b->ntasks,
                      b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN));
+  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN), 0);
   hipLaunchKernelGGL(k_batch_beta, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_NN), sc(b, S_SQ),
                      sc(b, S_SP), sc(b, S_BETA), hist_beta(b, j));
+  return LSQED_OK;
 }
 
 // Lanczos start: q = seeded random, p = 0, sq = 1/||q||, all problems active.
@@ -574,7 +703,7 @@ int start(lsqed_batch* b, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_zero, dim3(b->grid), dim3(kBThreads), 0, s, b->d_probs, b->d_tasks, b->ntasks, b->d_p);
   hipLaunchKernelGGL(k_batch_sumsq, dim3(b->grid), dim3(kBThreads), 0, s, b->d_probs, b->d_tasks, b->ntasks, b->d_q,
                      b->d_part);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN));
+  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN), 0);
   LSQED_HIP(hipGetLastError());
   std::vector<double> nn(np);
   LSQED_HIP(hipMemcpyAsync(nn.data(), sc(b, S_NN), np * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -599,20 +728,48 @@ void drop_graphs(lsqed_batch* b) {
   }
 }
 
-// Capture kSteps steps (q,p roles alternating) as graph `which`.
-int make_graph(lsqed_batch* b, int which) {
-  if (b->gexec[which]) return LSQED_OK;
-  LSQED_HIP(hipStreamBeginCapture(b->gs, hipStreamCaptureModeThreadLocal));
+// kSteps Lanczos steps (q,p roles alternating), the unit of a replay.
+int enqueue_replay_steps(lsqed_batch* b, bool accumulate, hipStream_t s) {
   for (int j = 0; j < kSteps; ++j) {
     double* q = (j & 1) ? b->d_p : b->d_q;
     double* p = (j & 1) ? b->d_q : b->d_p;
-    enqueue_step(b, q, p, j, which == 1, b->gs);
+    const int rc = enqueue_step(b, q, p, j, accumulate, s);
+    if (rc) return rc;
   }
+  return LSQED_OK;
+}
+
+// Capture a replay as graph `which` (0 plain, 1 with Ritz accumulation).
+// If the runtime cannot capture it, replays are enqueued eagerly instead.
+int make_graph(lsqed_batch* b, int which) {
+  if (b->gexec[which] || !b->use_graph) return LSQED_OK;
+  LSQED_HIP(hipStreamBeginCapture(b->gs, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_replay_steps(b, which == 1, b->gs);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->gs, &g);
-  if (e != hipSuccess) return fail(LSQED_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  if (rc || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    b->use_graph = false;
+    return LSQED_OK;
+  }
   b->graph[which] = g;
-  LSQED_HIP(hipGraphInstantiate(&b->gexec[which], g, nullptr, nullptr, 0));
+  if (hipGraphInstantiate(&b->gexec[which], g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    drop_graphs(b);
+    b->use_graph = false;
+  }
+  return LSQED_OK;
+}
+
+int run_replay(lsqed_batch* b, int which) {
+  if (b->use_graph && b->gexec[which]) {
+    LSQED_HIP(hipGraphLaunch(b->gexec[which], b->gs));
+    return LSQED_OK;
+  }
+  const int rc = enqueue_replay_steps(b, which == 1, b->gs);
+  if (rc) return rc;
+  LSQED_HIP(hipGetLastError());
   return LSQED_OK;
 }
 
@@ -675,9 +832,10 @@ int time_shape(lsqed_batch* b, float* ms) {
   hipEvent_t e0, e1;
   LSQED_HIP(hipEventCreate(&e0));
   LSQED_HIP(hipEventCreate(&e1));
-  enqueue_step(b, b->d_q, b->d_p, 0, false, b->gs);
+  if ((rc = enqueue_step(b, b->d_q, b->d_p, 0, false, b->gs))) return rc;
   LSQED_HIP(hipEventRecord(e0, b->gs));
-  for (int j = 1; j <= 4; ++j) enqueue_step(b, (j & 1) ? b->d_p : b->d_q, (j & 1) ? b->d_q : b->d_p, j, false, b->gs);
+  for (int j = 1; j <= 4; ++j)
+    if ((rc = enqueue_step(b, (j & 1) ? b->d_p : b->d_q, (j & 1) ? b->d_q : b->d_p, j, false, b->gs))) return rc;
   LSQED_HIP(hipEventRecord(e1, b->gs));
   LSQED_HIP(hipGetLastError());
   LSQED_HIP(hipEventSynchronize(e1));
@@ -689,20 +847,26 @@ int time_shape(lsqed_batch* b, float* ms) {
   return LSQED_OK;
 }
 
-// Autotune over rows per task x blocks per CU (one lane per state), then
-// lanes per state at the best tile; cached per batch shape.
+// Autotune, timed on the batch itself and cached per batch shape:
+//   chunk-table kernel: rows per task x blocks per CU at one lane per state,
+//   then 2 and 4 lanes per state at the best tile (or the fixed lane count);
+//   specialised row split: blocks per CU.
 int autotune(lsqed_batch* b) {
-  if (b->tuned) return LSQED_OK;
   const std::string key = shape_key(b);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       b->shape = it->second;
-      b->tuned = true;
       return LSQED_OK;
     }
   }
+  const int fixed_lps = b->opts.lanes_per_state;
+  const bool try_generic = b->opts.hv_kernel != 2;
+  const bool try_spec = b->opts.hv_kernel != 1 && fixed_lps == 0 && b->jit_ok;
+  if (!try_generic && !try_spec)
+    return fail(LSQED_ERR_ARG, "the specialised batched H.v is unavailable for this batch (N > 32, too many "
+                               "high bonds, or hipRTC disabled)");
   Shape best;
   float best_ms = 1e30f;
   auto trial = [&](Shape sh) -> int {
@@ -717,14 +881,23 @@ int autotune(lsqed_batch* b) {
     return LSQED_OK;
   };
   int rc;
-  for (int rows : {1024, 4096, 16384})
+  int rows_best = 4096;
+  if (try_generic) {
+    const int l0 = fixed_lps > 0 ? fixed_lps : 1;
+    for (int rows : {1024, 4096, 16384})
+      for (int bpc : {2, 4, 8})
+        if ((rc = trial(Shape{0, rows, l0, bpc}))) return rc;
+    rows_best = best.rows;
+    if (fixed_lps == 0) {
+      const Shape base = best;
+      for (int lps : {2, 4})
+        if ((rc = trial(Shape{0, base.rows, lps, base.bpc}))) return rc;
+    }
+  }
+  if (try_spec)
     for (int bpc : {2, 4, 8})
-      if ((rc = trial(Shape{rows, 1, bpc}))) return rc;
-  const Shape base = best;
-  for (int lps : {2, 4})
-    if ((rc = trial(Shape{base.rows, lps, base.bpc}))) return rc;
+      if ((rc = trial(Shape{1, rows_best, 1, bpc}))) return rc;
   b->shape = best;
-  b->tuned = true;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = best;
   return LSQED_OK;
@@ -737,10 +910,13 @@ int prepare(lsqed_batch* b, hipStream_t caller) {
   LSQED_HIP(hipEventRecord(e, caller));
   LSQED_HIP(hipStreamWaitEvent(b->gs, e, 0));
   (void)hipEventDestroy(e);
-  const Shape before = b->shape;
-  int rc = autotune(b);
+  int rc = setup_groups(b);
   if (rc) return rc;
-  if (before.rows != b->shape.rows || before.lps != b->shape.lps || before.bpc != b->shape.bpc) drop_graphs(b);
+  if ((rc = autotune(b))) return rc;
+  // graphs hold the launch arguments (task list, groups, grid): rebuild them
+  // every solve, since build flags and groups may have changed
+  drop_graphs(b);
+  b->use_graph = true;
   return upload_problems(b);  // also restores the chosen shape's task list and grid
 }
 
@@ -773,6 +949,10 @@ int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o,
   if (o && (o->lanes_per_state != 0 && o->lanes_per_state != 1 && o->lanes_per_state != 2 &&
             o->lanes_per_state != 4))
     return fail(LSQED_ERR_ARG, "lanes_per_state must be 0 (autotune), 1, 2 or 4");
+  if (o && (o->hv_kernel < 0 || o->hv_kernel > 2))
+    return fail(LSQED_ERR_ARG, "hv_kernel must be 0 (autotune), 1 (chunk table) or 2 (specialised row split)");
+  if (o && o->hv_kernel == 2 && o->lanes_per_state > 1)
+    return fail(LSQED_ERR_ARG, "lanes_per_state applies to the chunk-table kernel only");
   lsqed_batch* b = new lsqed_batch;
   if (o) b->opts = *o;
   if (b->opts.maxit <= 0) b->opts.maxit = 300;
@@ -790,10 +970,6 @@ int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o,
     smem = std::max(smem, sm);
   }
   b->smem = smem;
-  if (b->opts.lanes_per_state > 0) {
-    b->shape.lps = b->opts.lanes_per_state;
-    b->tuned = true;
-  }
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
@@ -808,7 +984,11 @@ int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o,
             hipMalloc((void**)&b->d_active, n * sizeof(int)) == hipSuccess &&
             hipMalloc((void**)&b->d_s, (size_t)S_COUNT * n * sizeof(double)) == hipSuccess &&
             hipMalloc((void**)&b->d_hist, (size_t)2 * kSteps * n * sizeof(double)) == hipSuccess &&
-            hipMalloc((void**)&b->d_obs, (size_t)kObsStride * n * sizeof(double)) == hipSuccess;
+            hipMalloc((void**)&b->d_obs, (size_t)kObsStride * n * sizeof(double)) == hipSuccess &&
+            hipMalloc((void**)&b->d_poff, n * sizeof(int64_t)) == hipSuccess &&
+            hipMalloc((void**)&b->d_pcoup, (size_t)4 * n * sizeof(double)) == hipSuccess &&
+            hipMalloc((void**)&b->d_pfields, (size_t)32 * n * sizeof(double)) == hipSuccess &&
+            hipMalloc((void**)&b->d_gidx, n * sizeof(int)) == hipSuccess;
   b->ring = b->opts.maxit / kSteps + 4;
   ok = ok && hipHostMalloc((void**)&b->h_hist, (size_t)2 * 2 * kSteps * n * sizeof(double), 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_active, (size_t)b->ring * n * sizeof(int), 0) == hipSuccess;
@@ -835,7 +1015,8 @@ int lsqed_batch_destroy(lsqed_batch* b) {
     if (e) (void)hipEventDestroy(e);
   if (b->gs) (void)hipStreamDestroy(b->gs);
   for (void* ptr : {(void*)b->d_q, (void*)b->d_p, (void*)b->d_y, (void*)b->d_psi, (void*)b->d_part, (void*)b->d_s,
-                    (void*)b->d_hist, (void*)b->d_obs, (void*)b->d_tasks, (void*)b->d_probs, (void*)b->d_active})
+                    (void*)b->d_hist, (void*)b->d_obs, (void*)b->d_tasks, (void*)b->d_probs, (void*)b->d_active,
+                    (void*)b->d_poff, (void*)b->d_pcoup, (void*)b->d_pfields, (void*)b->d_gidx, (void*)b->d_hvpart})
     (void)hipFree(ptr);
   if (b->h_hist) (void)hipHostFree(b->h_hist);
   if (b->h_active) (void)hipHostFree(b->h_active);
@@ -864,7 +1045,8 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream) {
   int slot = 0;                               // ring slot of the next activity upload
   std::vector<double> ritz;
   auto enqueue = [&](int r) -> int {
-    LSQED_HIP(hipGraphLaunch(b->gexec[0], b->gs));
+    const int rc2 = run_replay(b, 0);
+    if (rc2) return rc2;
     LSQED_HIP(hipMemcpyAsync(b->h_hist + (r & 1) * HS, b->d_hist, HS * sizeof(double), hipMemcpyDeviceToHost, b->gs));
     LSQED_HIP(hipEventRecord(b->ev[r & 1], b->gs));
     return LSQED_OK;
@@ -971,13 +1153,13 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream) {
       if (hipMemcpyAsync(b->d_active, ha + (size_t)r * np, np * sizeof(int), hipMemcpyHostToDevice, b->gs) ||
           hipMemcpyAsync(sc(b, S_W), hw + (size_t)r * kSteps * np, (size_t)kSteps * np * sizeof(double),
                          hipMemcpyHostToDevice, b->gs) ||
-          hipGraphLaunch(b->gexec[1], b->gs))
+          run_replay(b, 1))
         rc = fail(LSQED_ERR_HIP, "second Lanczos pass launch failed");
     }
     if (!rc) {
       hipLaunchKernelGGL(k_batch_sumsq, dim3(b->grid), dim3(kBThreads), 0, b->gs, b->d_probs, b->d_tasks, b->ntasks,
                          b->d_psi, b->d_part);
-      hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, b->gs, b->d_probs, np, b->d_part, sc(b, S_NN));
+      hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, b->gs, b->d_probs, np, b->d_part, sc(b, S_NN), 0);
       hipLaunchKernelGGL(k_batch_normalise, dim3(b->grid), dim3(kBThreads), 0, b->gs, b->d_probs, b->d_tasks,
                          b->ntasks, sc(b, S_NN), b->d_psi);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->gs) != hipSuccess)
@@ -1003,8 +1185,11 @@ int lsqed_batch_info(const lsqed_batch* b, int k, int* steps, double* resid, int
   return LSQED_OK;
 }
 
-int lsqed_batch_schedule(const lsqed_batch* b, int* rows_per_task, int* lanes_per_state, int* blocks_per_cu) {
+int lsqed_batch_schedule(const lsqed_batch* b, int* hv_kernel, int* rows_per_task, int* lanes_per_state,
+                         int* blocks_per_cu, int* graph) {
   if (!b) return fail(LSQED_ERR_ARG, "null handle");
+  if (hv_kernel) *hv_kernel = b->shape.kernel + 1;
+  if (graph) *graph = b->use_graph && b->gexec[0] ? 1 : 0;
   if (rows_per_task) *rows_per_task = b->shape.rows;
   if (lanes_per_state) *lanes_per_state = b->shape.lps;
   if (blocks_per_cu) *blocks_per_cu = b->shape.bpc;

@@ -44,6 +44,14 @@ struct JitArgs {
   const unsigned int* tasks;
   int ntasks;
   int offU[17];
+#if BATCH
+  const long long* poff;   // per problem: offset of its entries in x and y
+  const double* pcoup;     // per problem: J1/2, J2/2, D1/4, D2/4
+  const double* pfields;   // per problem: 32 fields
+  const int* active;       // per problem: still iterating
+  const int* gidx;         // the group's problems (batch indices)
+  int nprob;               // problems in the group
+#endif
 };
 
 template <bool DOT>
@@ -63,8 +71,10 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   for (int i = tid; i < 256; i += kThreads) s_RL0[i] = a.RL0[i];
   for (int i = tid; i < 9 * W1; i += kThreads) s_RL1[i] = a.RL1[i];
   for (int i = tid; i < (N + 1) * (NUP + 1); i += kThreads) s_binom[i] = a.binom[i];
+#if !BATCH
   if (DIAG_MODE == 1)
     for (int i = tid; i < N; i += kThreads) s_f[i] = a.fields[i];
+#endif
   __syncthreads();
   auto C = [&](int n, int k) -> unsigned int { return (k < 0 || k > NUP) ? 0u : s_binom[n * (NUP + 1) + k]; };
   auto rowbase = [&](unsigned int hh) -> unsigned int {
@@ -75,14 +85,40 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       if ((hh >> q) & 1u) { ++j; b += C(L + q, kl + j); }
     return b;
   };
-  constexpr double jh[2] = {JH0, JH1};
   constexpr int flip[2] = {FLIP0, FLIP1};
   const long long G = gridDim.x;
   const long long lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+#if BATCH
+  // tasks t = (problem of the group) * ntasks + (row-split task), dealt
+  // cyclically so that all blocks work on neighbouring problems together
+  const long long ntot = (long long)a.nprob * a.ntasks;
+  int loaded = -1;
+  for (long long t = lb; t < ntot; t += G) {
+    const int pl = (int)(t / a.ntasks);
+    const int st = (int)(t - (long long)pl * a.ntasks);
+    const int prob = a.gidx[pl];
+    if (!a.active[prob]) continue;  // block-uniform
+    const double* __restrict__ xv = a.x + a.poff[prob];
+    double* __restrict__ yv = a.y + a.poff[prob];
+    const double jh_[2] = {a.pcoup[4 * prob], a.pcoup[4 * prob + 1]};
+    const double dq0_ = a.pcoup[4 * prob + 2], dq1_ = a.pcoup[4 * prob + 3];
+    if (DIAG_MODE == 1 && prob != loaded) {
+      __syncthreads();
+      for (int i = tid; i < N; i += kThreads) s_f[i] = a.pfields[32 * prob + i];
+      loaded = prob;
+    }
+    double dotacc = 0.0;
+    const unsigned int task = a.tasks[st];
+#else
   const long long ta = lb * a.ntasks / G, tb = (lb + 1) * a.ntasks / G;
   double dotacc = 0.0;
   for (long long t = ta; t < tb; ++t) {
+    const double* __restrict__ xv = a.x;
+    double* __restrict__ yv = a.y;
+    constexpr double jh_[2] = {JH0, JH1};
+    constexpr double dq0_ = DQ0, dq1_ = DQ1;
     const unsigned int task = a.tasks[t];
+#endif
     const unsigned int h = task & 0xffffu;
     const int kl = NUP - __builtin_popcount(h);
     const int R = (int)C(L, kl);
@@ -110,7 +146,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     if (tid == 0) {
       int n = 0;
       for (int b = 0; b < NHIGH; ++b)
-        if (s_hbase[b] != 0xffffffffu) { s_hbase[n] = s_hbase[b]; s_hcoef[n] = jh[HIK[b]]; ++n; }
+        if (s_hbase[b] != 0xffffffffu) { s_hbase[n] = s_hbase[b]; s_hcoef[n] = jh_[HIK[b]]; ++n; }
       s_nact = n;
     }
     __syncthreads();
@@ -133,7 +169,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           v[u] = 0.0;
-          if (b0 + u < nact) v[u] = a.x[__builtin_amdgcn_readfirstlane(s_hbase[b0 + u]) + (unsigned int)i];
+          if (b0 + u < nact) v[u] = xv[__builtin_amdgcn_readfirstlane(s_hbase[b0 + u]) + (unsigned int)i];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -170,13 +206,13 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
               i2 = t0 + s_RL1[k0 * W1 + (l2 >> 8)];
             }
             const unsigned int idx = on ? cb[hp] + i2 : r;
-            v[u] = a.x[idx];
+            v[u] = xv[idx];
           }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int b = b0 + u;
-          if (b < NCROSS) acc += (((onm >> u) & 1u) ? jh[CRK[b]] : 0.0) * v[u];
+          if (b < NCROSS) acc += (((onm >> u) & 1u) ? jh_[CRK[b]] : 0.0) * v[u];
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -206,13 +242,13 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
               i2 = s_RL0[w0] + s_RL1[__builtin_popcount(w0) * W1 + (l2 >> 8)];
             }
             const unsigned int idx = on ? B + i2 : r;
-            v[u] = a.x[idx];
+            v[u] = xv[idx];
           }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int b = b0 + u;
-          if (b < NLOW) acc += (((onm >> u) & 1u) ? jh[LOWK[b]] : 0.0) * v[u];
+          if (b < NLOW) acc += (((onm >> u) & 1u) ? jh_[LOWK[b]] : 0.0) * v[u];
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -220,21 +256,37 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       const int c1 = NB1 - 2 * (cnt1 + cc1);
       double d = 0.0;
       if (DIAG_MODE == 0) {
-        if (DZ0 && c0 != 0) d = DQ0 * (double)c0;
-        if (DZ1 && c1 != 0) d = d + DQ1 * (double)c1;
+        if (DZ0 && c0 != 0) d = dq0_ * (double)c0;
+        if (DZ1 && c1 != 0) d = d + dq1_ * (double)c1;
       } else if (DIAG_MODE == 1) {
         double dis = 0.0;
 #pragma unroll
         for (int q = 0; q < L; ++q) dis = ((l >> q) & 1u) ? dis + s_f[q] : dis - s_f[q];
         dis += s_dish;
-        d = (DQ0 * (double)c0 + DQ1 * (double)c1) + 0.5 * dis;
+        d = (dq0_ * (double)c0 + dq1_ * (double)c1) + 0.5 * dis;
       }
-      const double xr = a.x[r];
+      const double xr = xv[r];
       const double yr = d * xr + acc;
-      a.y[r] = yr;
+      yv[r] = yr;
       if (DOT) dotacc += xr * yr;
     }
+#if BATCH
+    if (DOT) {
+      double v = dotacc;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      __syncthreads();
+      if ((tid & 63) == 0) s_red[tid >> 6] = v;
+      __syncthreads();
+      if (tid == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) tot += s_red[w];
+        a.dot[t] = tot;
+      }
+    }
+#endif
   }
+#if !BATCH
   if (DOT) {
     double v = dotacc;
 #pragma unroll
@@ -248,6 +300,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       a.dot[blockIdx.x] = tot;
     }
   }
+#endif
 }
 
 #ifndef WAVES_PER_EU
@@ -271,6 +324,17 @@ struct JitArgs {
   const uint32_t* tasks;
   int ntasks;
   int offU[17];
+};
+
+// Argument block of the batch variant (same leading layout).
+struct BatchJitArgs {
+  JitArgs base;
+  const int64_t* poff;
+  const double* pcoup;
+  const double* pfields;
+  const int* active;
+  const int* gidx;
+  int nprob;
 };
 
 struct JitModule {
@@ -302,7 +366,7 @@ static std::string dbl(double v) {
 // Source of the kernel specialised to the bond classes and couplings of `h`
 // (row split with L = min(16, N) low bits, see hv_split.hip).  Fields stay a
 // run-time array, so disorder realisations share one compiled kernel.
-std::string jit_source(const lsqed_ham* h) {
+std::string jit_source(const lsqed_ham* h, bool batch) {
   const int N = h->N, nup = h->nup, L = std::min(16, N), NH = N - L;
   std::vector<unsigned> lowm, him;
   std::vector<int> lowk, lowt, crq, crp, crk, hik;
@@ -325,13 +389,14 @@ std::string jit_source(const lsqed_ham* h) {
       }
     }
   std::ostringstream o;
-  o << "// generated by liblsqed (hv_jit.cpp)\n";
+  o << "// generated by liblsqed (hv_jit.cpp)\n#define BATCH " << (batch ? 1 : 0) << "\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
   const RowParams rp = make_row_params(h);
-  o << "constexpr double JH0 = " << dbl(rp.jh[0]) << ", JH1 = " << dbl(rp.jh[1]) << ", DQ0 = " << dbl(rp.dq[0])
-    << ", DQ1 = " << dbl(rp.dq[1]) << ";\n";
+  // the batch variant reads couplings per problem at run time
+  o << "constexpr double JH0 = " << dbl(batch ? 0.0 : rp.jh[0]) << ", JH1 = " << dbl(batch ? 0.0 : rp.jh[1])
+    << ", DQ0 = " << dbl(batch ? 0.0 : rp.dq[0]) << ", DQ1 = " << dbl(batch ? 0.0 : rp.dq[1]) << ";\n";
   o << "constexpr int FLIP0 = " << rp.flip[0] << ", FLIP1 = " << rp.flip[1] << ", DZ0 = " << rp.dz_on[0]
     << ", DZ1 = " << rp.dz_on[1] << ", DIAG_MODE = " << rp.diag_mode << ";\n";
   o << "constexpr int NLOW = " << lowm.size() << ", NCROSS = " << crq.size() << ", NHIGH = " << him.size()
@@ -404,7 +469,7 @@ bool jit_enabled() {
 
 int jit_prepare(lsqed_ham* h) {
   if (h->jit) return LSQED_OK;
-  const std::string src = jit_source(h);
+  const std::string src = jit_source(h, false);
   std::shared_ptr<JitModule> m;
   {
     std::lock_guard<std::mutex> lk(g_jit_mu);
@@ -442,6 +507,65 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
   void* args[] = {&a};
   LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, args,
+                                  nullptr));
+  return LSQED_OK;
+}
+
+// Batch variant: one module per bond structure and build mode (couplings and
+// fields per problem at run time).  *key receives the generated source, which
+// identifies the group of problems that can share the kernel.
+int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key) {
+  const std::string src = jit_source(h, true);
+  if (key) *key = src;
+  std::shared_ptr<JitModule> m;
+  {
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    auto it = g_jit_cache.find(src);
+    if (it != g_jit_cache.end()) {
+      m = it->second;
+    } else {
+      m = jit_compile(src);
+      g_jit_cache[src] = m;
+    }
+  }
+  if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the batched H.v failed: " + m->log);
+  *module = m.get();
+  return LSQED_OK;
+}
+
+int split_task_count(lsqed_ham* h, int* ntasks) {
+  SplitTables t;
+  const int rc = split_tables(h, t);
+  if (rc) return rc;
+  *ntasks = t.ntasks;
+  return LSQED_OK;
+}
+
+int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, int grid, hipStream_t s) {
+  SplitTables t;
+  int rc = split_tables(h0, t);
+  if (rc) return rc;
+  JitModule* m = (JitModule*)module;
+  BatchJitArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.base.x = L.x;
+  a.base.y = L.y;
+  a.base.dot = L.dot;
+  a.base.U = t.U;
+  a.base.RL0 = t.RL0;
+  a.base.RL1 = t.RL1;
+  a.base.binom = t.binom;
+  a.base.tasks = t.tasks;
+  a.base.ntasks = t.ntasks;
+  for (int k = 0; k < 17; ++k) a.base.offU[k] = t.offU[k];
+  a.poff = L.poff;
+  a.pcoup = L.pcoup;
+  a.pfields = L.pfields;
+  a.active = L.active;
+  a.gidx = L.gidx;
+  a.nprob = L.nprob;
+  void* args[] = {&a};
+  LSQED_HIP(hipModuleLaunchKernel(L.dot ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, args,
                                   nullptr));
   return LSQED_OK;
 }

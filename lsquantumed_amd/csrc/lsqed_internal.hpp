@@ -135,6 +135,23 @@ bool jit_enabled();
 int jit_prepare(lsqed_ham* h);
 int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
 
+// Batch variant of the specialised kernel (hv_jit.cpp): problems sharing a
+// bond structure and build mode form a group; one launch covers the group.
+struct BatchJitLaunch {
+  const double* x;
+  double* y;
+  double* dot;            // per (problem of the group, row-split task) partial of x.y, or null
+  const int64_t* poff;    // per batch problem
+  const double* pcoup;    // per batch problem: jh0, jh1, dq0, dq1
+  const double* pfields;  // per batch problem: 32 doubles
+  const int* active;      // per batch problem
+  const int* gidx;        // batch indices of the group's problems
+  int nprob;              // problems in the group
+};
+int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key);
+int split_task_count(lsqed_ham* h, int* ntasks);
+int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, int grid, hipStream_t s);
+
 // Reductions (krylov.hip)
 int device_dot(lsqed_ham* h, const double* x, const double* y, int64_t n, double* out_host, hipStream_t s);
 

@@ -81,18 +81,24 @@ def test_batch_disorder_sample_vs_plain(O, torch):
     assert abs(e0[0] - ref0) <= 1e-9 * abs(ref0)
 
 
-def test_batch_lanes_per_state_agree(torch):
-    """Fixed lanes-per-state schedules (1, 2, 4) and the autotuned one give
-    the same E0 (the Lanczos recurrence is the same up to rounding)."""
+def test_batch_schedules_agree(torch):
+    """Every H.v schedule gives the same E0 (the Lanczos recurrence is the
+    same up to rounding): the chunk-table kernel at 1, 2 and 4 lanes per
+    state, the row split specialised per lattice group, and the autotuned
+    choice; N = 18 and 20 exercise the specialised kernel's high bits."""
     from lsquantumed_amd.batch import BatchSolver, disorder_chains
-    hams = disorder_chains([12, 14], [3, 3], J1=1.0, J2=0.2, W=2.0, seed=7)
+    hams = disorder_chains([12, 18, 20], [3, 2, 2], J1=1.0, J2=0.2, W=2.0, seed=7)
     vals = []
-    for lps in (0, 1, 2, 4):
-        say("lanes", lps)
-        b = BatchSolver(hams, tol=1e-12, lanes_per_state=lps)
+    for kern, lps in ((0, 0), (1, 1), (1, 2), (1, 4), (2, 0)):
+        say("schedule", kern, lps)
+        b = BatchSolver(hams, tol=1e-12, lanes_per_state=lps, hv_kernel=kern)
         vals.append(b.solve())
+        sch = b.schedule()
+        say("  ", sch)
         if lps:
-            assert b.schedule()["lanes_per_state"] == lps
+            assert sch["lanes_per_state"] == lps and sch["hv_kernel"] == "chunk table"
+        if kern == 2:
+            assert sch["hv_kernel"] == "specialised row split"
         b.destroy()
     for v in vals[1:]:
         np.testing.assert_allclose(v, vals[0], rtol=1e-12, atol=0)
@@ -127,6 +133,7 @@ def test_batch_vectors_and_observables(O, torch):
     hams = disorder_chains([10, 12], [2, 3], J1=J1, J2=J2, W=1.0, seed=11)
     b = BatchSolver(hams, tol=1e-12, keep_vectors=True)
     e0 = b.solve()
+    say("schedule", b.schedule())
     obs = b.observables()
     for k, h in enumerate(hams):
         N = h.basis.nspins
