@@ -73,6 +73,107 @@ def cpu_baseline(h_args, target_s=12.0):
             "rows_per_s": round(rows_per_s, 1)}
 
 
+def disorder_batch(reps: int = 3):
+    """Batched disorder ground states against the plain one-at-a-time path.
+
+    Workload: 64 periodic chains of N=18, 64 of N=20 and 128 of N=22 (Sz=0
+    dims 48,620 / 184,756 / 705,432), J1=D1=1, J2=D2=0.2, fields
+    h_i ~ U[-2, 2] from mt19937_64(1 + k) for the k-th realisation, Lanczos to
+    a 1e-9 relative residual.  Batched: one BatchSolver over all 256 (warm-up
+    solve = autotune + kernel compilation, then `reps` timed solves).  Plain:
+    the realisation loop of examples/main_disorder.cpp:82-120 on the GPU --
+    one model per N, build_disorder, SolverInit(nev=1), solve, solution,
+    SolverClean per realisation (Krylov-Schur, tol 1e-9), after one untimed
+    solve per N.  Also timed: the batched solver fed one realisation at a
+    time (batch size 1)."""
+    import numpy as np
+    import torch
+
+    import lsquantumed_amd as Q
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+
+    sizes, counts = [18, 20, 22], [64, 64, 128]
+    hams = disorder_chains(sizes, counts, J1=1.0, J2=0.2, W=2.0, seed=1)
+    n = len(hams)
+    b = BatchSolver(hams, tol=1e-9)
+    t = time.perf_counter()
+    b.solve()
+    first = time.perf_counter() - t
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        e0 = b.solve()
+        times.append(time.perf_counter() - t)
+    t_b = float(np.mean(times))
+    info = [b.info(k) for k in range(n)]
+    sched = b.schedule()
+    b.destroy()
+
+    # plain: one model per N reused, fields rebuilt per realisation
+    per_n = {}
+    for N in sizes:
+        env = Q.Environment(N)
+        h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.chain1D(env), 1.0, 1.0, 0.2, 0.2)
+        h.build_diag(env)
+        h.build_off_diag(env)
+        h.build_disorder(env, np.zeros(N))
+        eps = Q.Solver.SolverInit(h, 1, tol=1e-9)
+        Q.Solver.solve(env, eps)
+        Q.Solver.SolverClean(eps)
+        per_n[N] = (env, h)
+    plain = np.zeros(n)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k, hk in enumerate(hams):
+        env, h = per_n[hk.basis.nspins]
+        h.build_disorder(env, hk.hi)
+        eps = Q.Solver.SolverInit(h, 1, tol=1e-9)
+        if Q.Solver.solve(env, eps) < 1:
+            raise RuntimeError(f"plain solve of realisation {k} did not converge")
+        plain[k], _ = Q.Solver.solution(eps, 0)
+        Q.Solver.SolverClean(eps)
+    t_p = time.perf_counter() - t
+
+    # the batched solver one realisation at a time
+    singles = [BatchSolver([h], tol=1e-9) for h in hams]
+    for N in sizes:  # warm-up per lattice (tuning, compilation)
+        next(s for s, h in zip(singles, hams) if h.basis.nspins == N).solve()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for s in singles:
+        s.solve()
+    t_1 = time.perf_counter() - t
+    for s in singles:
+        s.destroy()
+
+    rel = np.abs(e0 - plain) / np.abs(plain)
+    # algorithmic bytes of one batched step with every model active: H.v
+    # (x read, y written, one 8 B gather per off-diagonal nonzero) + update
+    # (y, q, p read, p written)
+    hv_bytes = sum(algorithmic_bytes(N, N // 2, 2 * N)[0] * c for N, c in zip(sizes, counts))
+    upd_bytes = 32 * sum(math.comb(N, N // 2) * c for N, c in zip(sizes, counts))
+    return {
+        "workload": "64x N=18 + 64x N=20 + 128x N=22 periodic chains, J1=D1=1, J2=D2=0.2, h_i~U[-2,2] from "
+                    "mt19937_64(1+k), Sz=0, ground state to 1e-9 relative residual",
+        "realisations": n,
+        "batched_realisations_per_s": round(n / t_b, 2),
+        "batched_solve_s": round(t_b, 4),
+        "batched_first_solve_s": round(first, 3),
+        "lanczos_steps_mean": round(float(np.mean([i[0] for i in info])), 1),
+        "all_converged": bool(all(i[2] for i in info)),
+        "schedule": sched,
+        "step_algorithmic_bytes": hv_bytes + upd_bytes,
+        "step_gbs": round((hv_bytes + upd_bytes) / (sched["step_ms"] * 1e-3) / 1e9, 1) if sched["step_ms"] else None,
+        "plain_realisations_per_s": round(n / t_p, 2),
+        "plain_s": round(t_p, 3),
+        "plain": "Krylov-Schur nev=1 per realisation (SolverInit/solve/solution/SolverClean, tol 1e-9)",
+        "batch_of_one_realisations_per_s": round(n / t_1, 2),
+        "speedup_vs_plain": round(t_p / t_b, 2),
+        "max_rel_e0_diff_vs_plain": float(rel.max()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +183,9 @@ def main():
     ap.add_argument("--lx", type=int, default=4)
     ap.add_argument("--ly", type=int, default=8)
     ap.add_argument("--hv-kernel", type=int, default=0, help="0 auto, 1 generic, 2 row split, 3 specialised row split")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-disorder measurement")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched-disorder measurement (one JSON object; used for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Sz-sector path here is built and measured on a single "
@@ -93,6 +197,9 @@ def main():
     import lsquantumed_amd as Q
 
     torch.cuda.set_device(0)
+    if args.batch_only:
+        print(json.dumps({"batched_disorder": disorder_batch()}))
+        return
     lx, ly = args.lx, args.ly
     N = lx * ly
     J1 = D1 = 1.0
@@ -153,6 +260,9 @@ def main():
     step_ms = ev0.elapsed_time(ev1) / args.steps
     steps_per_s = 1e3 / step_ms
 
+    del q, p, y
+    torch.cuda.empty_cache()
+    batch = None if args.no_batch else disorder_batch()
     cpu = None if args.no_cpu else cpu_baseline((N, N // 2, J1, D1, J2, D2, lx, ly, B))
     out = {
         "metric": METRIC,
@@ -177,6 +287,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(hv_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name, lx, ly)},
         "cpu_baseline": cpu,
+        "batched_disorder": batch,
         "alpha_last": alphas[-1],
     }
     print(json.dumps(out))

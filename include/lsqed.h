@@ -237,10 +237,11 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream);
 int lsqed_batch_info(const lsqed_batch* b, int k, int* steps, double* resid, int* converged);
 /* Launch shape in use (the autotuner's choice): H.v kernel (1 chunk table,
  * 2 specialised row split), rows per task, lanes per basis state, blocks per
- * CU, and whether replays run from a captured hipGraph (1) or are enqueued
- * launch by launch (0).  Any pointer may be NULL. */
+ * CU, whether replays run from a captured hipGraph (1) or are enqueued
+ * launch by launch (0), and the tuner's time of one Lanczos step of the
+ * whole batch with every model active (ms).  Any pointer may be NULL. */
 int lsqed_batch_schedule(const lsqed_batch* b, int* hv_kernel, int* rows_per_task, int* lanes_per_state,
-                         int* blocks_per_cu, int* graph);
+                         int* blocks_per_cu, int* graph, double* step_ms);
 /* Unit-norm ground state of model k into psi (D_k doubles, device). */
 int lsqed_batch_ground_state(lsqed_batch* b, int k, double* psi, lsqed_stream_t stream);
 /* Hellmann-Feynman derivatives of every model's E0 from its ground state,

@@ -77,7 +77,7 @@ SIGNATURES = {
     "lsqed_batch_create": (_i32, [_i32, _P(_vp), _P(BatchOpts), _P(_vp)]),
     "lsqed_batch_solve": (_i32, [_vp, _vp, _vp]),
     "lsqed_batch_info": (_i32, [_vp, _i32, _P(_i32), _P(_dbl), _P(_i32)]),
-    "lsqed_batch_schedule": (_i32, [_vp, _P(_i32), _P(_i32), _P(_i32), _P(_i32), _P(_i32)]),
+    "lsqed_batch_schedule": (_i32, [_vp, _P(_i32), _P(_i32), _P(_i32), _P(_i32), _P(_i32), _P(_dbl)]),
     "lsqed_batch_ground_state": (_i32, [_vp, _i32, _vp, _vp]),
     "lsqed_batch_observables": (_i32, [_vp, _vp, _vp]),
     "lsqed_batch_destroy": (_i32, [_vp]),

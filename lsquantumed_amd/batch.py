@@ -78,10 +78,12 @@ class BatchSolver:
     def schedule(self) -> dict:
         """Launch shape in use (the autotuner's choice)."""
         k, r, l, g, gr = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(lib().lsqed_batch_schedule(self.handle, C.byref(k), C.byref(r), C.byref(l), C.byref(g), C.byref(gr)))
+        ms = C.c_double()
+        check(lib().lsqed_batch_schedule(self.handle, C.byref(k), C.byref(r), C.byref(l), C.byref(g), C.byref(gr),
+                                         C.byref(ms)))
         return {"hv_kernel": {1: "chunk table", 2: "specialised row split"}.get(k.value, "?"),
                 "rows_per_task": r.value, "lanes_per_state": l.value, "blocks_per_cu": g.value,
-                "graph": bool(gr.value)}
+                "graph": bool(gr.value), "step_ms": round(ms.value, 4)}
 
     def ground_state(self, k: int, out=None, stream=None):
         import torch

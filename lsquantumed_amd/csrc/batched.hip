@@ -442,6 +442,7 @@ struct Shape {
   int rows = 4096;  // rows per task
   int lps = 1;      // lanes per basis state (kernel 0)
   int bpc = 4;      // blocks per CU
+  float step_ms = 0.0f;  // one Lanczos step of the whole batch (all problems active), as timed by the tuner
   bool operator!=(const Shape& o) const { return kernel != o.kernel || rows != o.rows || lps != o.lps || bpc != o.bpc; }
 };
 
@@ -877,6 +878,7 @@ int autotune(lsqed_batch* b) {
     if (ms < best_ms) {
       best_ms = ms;
       best = sh;
+      best.step_ms = ms;
     }
     return LSQED_OK;
   };
@@ -1186,8 +1188,9 @@ int lsqed_batch_info(const lsqed_batch* b, int k, int* steps, double* resid, int
 }
 
 int lsqed_batch_schedule(const lsqed_batch* b, int* hv_kernel, int* rows_per_task, int* lanes_per_state,
-                         int* blocks_per_cu, int* graph) {
+                         int* blocks_per_cu, int* graph, double* step_ms) {
   if (!b) return fail(LSQED_ERR_ARG, "null handle");
+  if (step_ms) *step_ms = b->shape.step_ms;
   if (hv_kernel) *hv_kernel = b->shape.kernel + 1;
   if (graph) *graph = b->use_graph && b->gexec[0] ? 1 : 0;
   if (rows_per_task) *rows_per_task = b->shape.rows;

@@ -306,10 +306,17 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #ifndef WAVES_PER_EU
 #define WAVES_PER_EU 4
 #endif
+#if BATCH
+#define KNAME_NODOT hv_bjit_nodot
+#define KNAME_DOT hv_bjit_dot
+#else
+#define KNAME_NODOT hv_jit_nodot
+#define KNAME_DOT hv_jit_dot
+#endif
 extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
-void hv_jit_nodot(JitArgs a) { hv_body<false>(a); }
+void KNAME_NODOT(JitArgs a) { hv_body<false>(a); }
 extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
-void hv_jit_dot(JitArgs a) { hv_body<true>(a); }
+void KNAME_DOT(JitArgs a) { hv_body<true>(a); }
 )JIT";
 
 struct JitArgs {
@@ -413,7 +420,7 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   return o.str();
 }
 
-static std::shared_ptr<JitModule> jit_compile(const std::string& src) {
+static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch) {
   auto m = std::make_shared<JitModule>();
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "lsqed_hv_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
@@ -443,8 +450,8 @@ static std::shared_ptr<JitModule> jit_compile(const std::string& src) {
     m->log += "\nhipModuleLoadData failed";
     return m;
   }
-  if (hipModuleGetFunction(&m->fn_dot, m->mod, "hv_jit_dot") != hipSuccess ||
-      hipModuleGetFunction(&m->fn_nodot, m->mod, "hv_jit_nodot") != hipSuccess) {
+  if (hipModuleGetFunction(&m->fn_dot, m->mod, batch ? "hv_bjit_dot" : "hv_jit_dot") != hipSuccess ||
+      hipModuleGetFunction(&m->fn_nodot, m->mod, batch ? "hv_bjit_nodot" : "hv_jit_nodot") != hipSuccess) {
     m->log += "\nhipModuleGetFunction failed";
     return m;
   }
@@ -477,7 +484,7 @@ int jit_prepare(lsqed_ham* h) {
     if (it != g_jit_cache.end()) {
       m = it->second;
     } else {
-      m = jit_compile(src);
+      m = jit_compile(src, false);
       g_jit_cache[src] = m;
     }
   }
@@ -524,7 +531,7 @@ int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key) {
     if (it != g_jit_cache.end()) {
       m = it->second;
     } else {
-      m = jit_compile(src);
+      m = jit_compile(src, true);
       g_jit_cache[src] = m;
     }
   }
