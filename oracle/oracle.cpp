@@ -466,6 +466,54 @@ void oq_hv_matfree(void* h, const double* hi, const double* x, double* y, int64_
   }
 }
 
+// Rows of H x for a vector given by a formula of the rank (no D-sized array:
+// for sectors whose vectors do not fit in host memory).  The formula has
+// period 1000003 * 999983 (> 2^39), so a rank that is wrong by any amount
+// up to that (a 32-bit truncation included) reads a different value:
+//   x(c) = (((c mod P1) K1 mod P1) + ((c mod P2) K2 mod P2)) * 2^-20 - 1,
+// exact in floating point (both terms < 2^20), so any implementation
+// reproduces it bit for bit.
+double oq_formula_x(int64_t c) {
+  const int64_t P1 = 1000003, P2 = 999983, K1 = 7919, K2 = 104729;
+  const int64_t n = ((c % P1) * K1) % P1 + ((c % P2) * K2) % P2;
+  return (double)n * (1.0 / 1048576.0) - 1.0;
+}
+
+void oq_hv_rows_formula(void* h, const double* hi, double* y, int64_t row0, int64_t nrows, int nthreads) {
+  Model& m = *(Model*)h;
+  const BinomTable& T = btab();
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  double jh[2] = {std::fabs(m.J[0]) >= 1e-14 ? m.J[0] * 0.5 : 0.0, std::fabs(m.J[1]) >= 1e-14 ? m.J[1] * 0.5 : 0.0};
+  double dq[2] = {std::fabs(m.Dz[0]) >= 1e-14 ? m.Dz[0] * 0.25 : 0.0,
+                  std::fabs(m.Dz[1]) >= 1e-14 ? m.Dz[1] * 0.25 : 0.0};
+  if (hi) { dq[0] = m.Dz[0] * 0.25; dq[1] = m.Dz[1] * 0.25; }
+#pragma omp parallel
+  {
+    int nt = omp_get_num_threads(), t = omp_get_thread_num();
+    int64_t a = row0 + nrows * t / nt, b = row0 + nrows * (t + 1) / nt;
+    if (a < b) {
+      uint64_t s = unrank_ref(a, m.N, m.nup);
+      for (int64_t r = a; r < b; ++r) {
+        double diag = 0.0, acc = 0.0;
+        for (int k = 0; k < 2; ++k) {
+          int64_t c = 0;
+          for (size_t q = 0; q < m.bi[k].size(); ++q) {
+            uint64_t mask = (1ull << m.bi[k][q]) | (1ull << m.bj[k][q]);
+            uint64_t both = s & mask;
+            if (both == 0 || both == mask) { ++c; continue; }
+            --c;
+            if (jh[k] != 0.0) acc += jh[k] * oq_formula_x((int64_t)rank_exact(s ^ mask, T));
+          }
+          diag += dq[k] * (double)c;
+        }
+        if (hi) diag += 0.5 * disorder_elem(s, m.N, hi);
+        y[r - row0] = diag * oq_formula_x(r) + acc;
+        s = gosper_next(s);
+      }
+    }
+  }
+}
+
 // --- Observables (src/spinOperators/correlation.cpp, spinOPft.tmpl_func.hpp) --
 
 // SzSz::ExpectVal: <psi| 0.25*(+1 parallel / -1 antiparallel) |psi>

@@ -90,6 +90,8 @@ def lib():
             "oq_csr_free": (None, [vp]),
             "oq_csr_spmv": (None, [vp, vp, vp, i32]),
             "oq_hv_matfree": (None, [vp, vp, vp, vp, i64, i64, i32]),
+            "oq_hv_rows_formula": (None, [vp, vp, vp, i64, i64, i32]),
+            "oq_formula_x": (C.c_double, [i64]),
             "oq_szsz": (dbl, [i32, i32, vp, i32, i32]),
             "oq_spsm": (dbl, [i32, i32, vp, i32, i32, i32]),
             "oq_sqz_apply": (None, [i32, i32, vp, i32, vp]),
@@ -252,6 +254,28 @@ class Model:
             hp = _p(hi)
         lib().oq_hv_matfree(self.h, hp, _p(x), _p(y), row0, nrows, nthreads)
         return y
+
+    def hv_rows_formula(self, row0, nrows, hi=None, nthreads=0):
+        """Rows [row0, row0+nrows) of H x with x(c) = formula_x(c)."""
+        y = np.zeros(nrows)
+        hp = None
+        if hi is not None:
+            hi = np.ascontiguousarray(hi, np.float64)
+            hp = _p(hi)
+        lib().oq_hv_rows_formula(self.h, hp, _p(y), row0, nrows, nthreads)
+        return y
+
+
+FORMULA_P = ((1000003, 7919), (999983, 104729))
+
+
+def formula_x(c: np.ndarray) -> np.ndarray:
+    """x(c) = (((c mod P1) K1 mod P1) + ((c mod P2) K2 mod P2)) * 2^-20 - 1,
+    the vector of oq_hv_rows_formula (exact in floating point)."""
+    c = np.asarray(c, np.int64)
+    (p1, k1), (p2, k2) = FORMULA_P
+    n = (c % p1) * k1 % p1 + (c % p2) * k2 % p2
+    return n.astype(np.float64) * (1.0 / 1048576.0) - 1.0
 
 
 # --- eigensolve (stands in for SLEPc EPS, src/solver.cpp:7-68) -------------

@@ -224,3 +224,44 @@ def test_hv_4x8_properties_and_rows(O, torch, kernel):
     assert abs(a - b) <= 1e-10 * (abs(a) + abs(b))
     w = h.matvec(2.0 * x - 3.0 * z)
     assert torch.max(torch.abs(w - (2.0 * y - 3.0 * hz))).item() <= 1e-11 * torch.max(torch.abs(w)).item()
+
+
+def formula_vector(O, torch, D, chunk=1 << 27):
+    """x(c) of the oracle's formula vector, built on the GPU in chunks (exact:
+    integer arithmetic, then a power-of-two scale)."""
+    (p1, k1), (p2, k2) = O.FORMULA_P
+    x = torch.empty(D, dtype=torch.float64, device="cuda")
+    for s in range(0, D, chunk):
+        c = torch.arange(s, min(D, s + chunk), dtype=torch.int64, device="cuda")
+        n = (c % p1) * k1 % p1 + (c % p2) * k2 % p2
+        x[s:s + c.numel()] = n.double() * (1.0 / 1048576.0) - 1.0
+        del c, n
+    return x
+
+
+def test_hv_6x6_rows_formula_vector(O, torch):
+    """6x6 J1-J2 (N = 36, D = 9,075,135,300; 72.6 GB per vector): rows of
+    H x sampled across the sector (64-bit ranks) against the oracle, for x
+    given by a formula of the rank that the oracle evaluates itself."""
+    import time
+    h = make("square", 36, 0, 1.0, 1.0, 0.5, 0.5, 6, 6)
+    D = h.hamilt_size()
+    assert D == 9075135300
+    say("hv 6x6: kernel", h.hv_kernel())
+    x = formula_vector(O, torch, D)
+    xs = x[[0, 1, 2**32 + 7, D - 1]].cpu().numpy()
+    assert np.array_equal(xs, O.formula_x(np.array([0, 1, 2**32 + 7, D - 1])))
+    y = torch.empty_like(x)
+    h.matvec(x, y)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    h.matvec(x, y)
+    torch.cuda.synchronize()
+    say("hv 6x6 ms", round((time.perf_counter() - t) * 1e3, 1))
+    m = oracle_model(O, "square", 36, 0, 1.0, 1.0, 0.5, 0.5, 6, 6)
+    for row0 in (0, 2**32 - 1000, D // 2, D - 2000):
+        yref = m.hv_rows_formula(row0, 2000)
+        got = y[row0:row0 + 2000].cpu().numpy()
+        assert np.max(np.abs(got - yref)) <= 1e-12 * np.abs(yref).max(), row0
+    del x, y
+    torch.cuda.empty_cache()

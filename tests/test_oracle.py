@@ -300,3 +300,21 @@ def test_dsf_fixture(O, literals, refdata, which):
             W = sum(O.dsf_weight(a[n], q, coords) for n in grp)
             Wref = sum(O.dsf_weight(ref_a[n], q, coords) for n in grp)
             assert abs(W - Wref) < 1e-10, (grp, qx, W, Wref)
+
+
+def test_formula_vector_rows(O):
+    """oq_hv_rows_formula (H x for x given by a formula of the rank, used for
+    sectors too large for a host vector) equals the matrix-free product with
+    the same x stored, and the numpy formula equals the C one."""
+    c = np.array([0, 1, 2, 999982, 999983, 1000002, 1000003, 2**32, 2**32 + 1, 9075135299], np.int64)
+    assert np.array_equal(O.formula_x(c), np.array([O.lib().oq_formula_x(int(v)) for v in c]))
+    # period > 2^39: a 32-bit truncation of the rank reads a different value
+    assert O.formula_x(np.array([2**32 + 5]))[0] != O.formula_x(np.array([5]))[0]
+    hi = np.random.default_rng(3).uniform(-1, 1, 12)
+    m = O.Model(O.SQUARE, 12, 6, 1.0, 0.7, 0.4, 0.3, lx=3, ly=4)
+    x = O.formula_x(np.arange(m.D))
+    for h in (None, hi):
+        ref = m.hv_matfree(x, hi=h)
+        got = m.hv_rows_formula(0, m.D, hi=h)
+        assert np.array_equal(got, ref)
+        assert np.array_equal(m.hv_rows_formula(100, 50, hi=h), ref[100:150])
