@@ -103,11 +103,11 @@ int lsqed_ham_set_couplings(lsqed_ham* h, double J1, double D1, double J2, doubl
  * on the device, must not alias. */
 int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
 
-/* H.v kernel choice: 0 = automatic (specialised row split when N <= 32 and
- * hipRTC is available, else the row split, else the generic chunk-table
- * kernel), 1 = generic chunk-table kernel, 2 = row-split kernel, 3 = row
- * split specialised to the lattice (compiled once per bond structure with
- * hipRTC).  No counterpart in the reference (whose operator is an assembled
+/* H.v kernel choice: 0 = automatic (specialised row split when N <= 40 and
+ * hipRTC is available, else the row split when N <= 32, else the generic
+ * chunk-table kernel), 1 = generic chunk-table kernel, 2 = row-split kernel
+ * (N <= 32), 3 = row split specialised to the lattice (compiled once per
+ * bond structure with hipRTC; 64-bit row bases above N = 32).  No counterpart in the reference (whose operator is an assembled
  * matrix); used by the tests to check every kernel and by the tuner.
  * Setting LSQED_JIT=0 in the environment keeps automatic mode off hipRTC. */
 int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant);

@@ -297,15 +297,12 @@ static int launch_hv_t(lsqed_ham* h, const double* x, double* y, double* dot_par
 // Kernel choice for H.v: 3 = specialised row split (hipRTC), 2 = row split,
 // 1 = generic chunk table.
 static int hv_kind(lsqed_ham* h) {
-  if (h->hv_variant == 1) return 1;
-  if (h->hv_variant == 2) return 2;
-  if (!split_supported(h)) return 1;
-  if (h->hv_variant == 3) return 3;
+  if (h->hv_variant >= 1) return h->hv_variant;
   if (jit_enabled() && !h->jit_failed && split_jit_fits(h)) {
     if (jit_prepare(h) == LSQED_OK) return 3;
-    h->jit_failed = true;  // keep the unspecialised row split
+    h->jit_failed = true;  // keep an unspecialised kernel
   }
-  return 2;
+  return split_supported(h) ? 2 : 1;
 }
 
 int hv_launch_grid(lsqed_ham* h) {
@@ -510,9 +507,10 @@ int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant) {
   if (!h) return fail(LSQED_ERR_ARG, "null handle");
   if (variant < 0 || variant > 3)
     return fail(LSQED_ERR_ARG, "variant must be 0 (auto), 1 (generic), 2 (row split) or 3 (specialised row split)");
-  if (variant >= 2 && !split_supported(h)) return fail(LSQED_ERR_ARG, "the row-split kernels need N <= 32");
+  if (variant == 2 && !split_supported(h)) return fail(LSQED_ERR_ARG, "the unspecialised row split needs N <= 32");
   if (variant == 3) {
-    if (!split_jit_fits(h)) return fail(LSQED_ERR_ARG, "more than 64 bonds between high sites");
+    if (!split_jit_fits(h))
+      return fail(LSQED_ERR_ARG, "the specialised row split needs N <= 40 and at most 256 bonds between high sites");
     int rc = jit_prepare(h);
     if (rc) return rc;
   }

@@ -32,6 +32,22 @@ constexpr int kThreads = 256;
 constexpr int kPerLane = 8;
 constexpr int kChunk = kThreads * kPerLane;
 
+#if WIDE
+typedef unsigned long long BT;  // row bases and ranks (D >= 2^32)
+#else
+typedef unsigned int BT;
+#endif
+
+__device__ __forceinline__ BT rfl(BT v) {
+#if WIDE
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((BT)hi << 32) | lo;
+#else
+  return __builtin_amdgcn_readfirstlane(v);
+#endif
+}
+
 struct JitArgs {
   const double* x;
   double* y;
@@ -41,6 +57,7 @@ struct JitArgs {
   const unsigned short* RL0;
   const unsigned short* RL1;
   const unsigned int* binom;
+  const unsigned long long* binom64;
   const unsigned int* tasks;
   int ntasks;
   int offU[17];
@@ -58,28 +75,33 @@ template <bool DOT>
 __device__ __forceinline__ void hv_body(const JitArgs& a) {
   __shared__ unsigned short s_RL0[256];
   __shared__ unsigned short s_RL1[9 * W1];
-  __shared__ unsigned int s_binom[(N + 1) * (NUP + 1)];
+  __shared__ BT s_binom[(N + 1) * (NUP + 1)];
   __shared__ double s_f[N];
-  __shared__ unsigned int s_hbase[NHIGH + 1];
+  __shared__ BT s_hbase[NHIGH + 1];
   __shared__ double s_hcoef[NHIGH + 1];
-  __shared__ unsigned int s_cbase[NH + 1];
-  __shared__ unsigned int s_B;
+  __shared__ BT s_cbase[NH + 1];
+  __shared__ BT s_B;
   __shared__ int s_nact, s_cnt0, s_cnt1;
   __shared__ double s_dish;
   __shared__ double s_red[kThreads / 64];
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += kThreads) s_RL0[i] = a.RL0[i];
   for (int i = tid; i < 9 * W1; i += kThreads) s_RL1[i] = a.RL1[i];
+#if WIDE
+  for (int i = tid; i < (N + 1) * (NUP + 1); i += kThreads) s_binom[i] = a.binom64[i];
+#else
   for (int i = tid; i < (N + 1) * (NUP + 1); i += kThreads) s_binom[i] = a.binom[i];
+#endif
 #if !BATCH
   if (DIAG_MODE == 1)
     for (int i = tid; i < N; i += kThreads) s_f[i] = a.fields[i];
 #endif
   __syncthreads();
-  auto C = [&](int n, int k) -> unsigned int { return (k < 0 || k > NUP) ? 0u : s_binom[n * (NUP + 1) + k]; };
-  auto rowbase = [&](unsigned int hh) -> unsigned int {
+  constexpr BT kNone = ~(BT)0;
+  auto C = [&](int n, int k) -> BT { return (k < 0 || k > NUP) ? (BT)0 : s_binom[n * (NUP + 1) + k]; };
+  auto rowbase = [&](unsigned int hh) -> BT {
     const int kl = NUP - __builtin_popcount(hh);
-    unsigned int b = 0;
+    BT b = 0;
     int j = 0;
     for (int q = 0; q < NH; ++q)
       if ((hh >> q) & 1u) { ++j; b += C(L + q, kl + j); }
@@ -119,15 +141,17 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     constexpr double dq0_ = DQ0, dq1_ = DQ1;
     const unsigned int task = a.tasks[t];
 #endif
-    const unsigned int h = task & 0xffffu;
+    const unsigned int h = task & 0xffffffu;
     const int kl = NUP - __builtin_popcount(h);
     const int R = (int)C(L, kl);
-    const int i0 = (int)(task >> 16) * kChunk;
+    const int i0 = (int)(task >> 24) * kChunk;
     __syncthreads();
-    if (tid < NHIGH) {
-      const bool anti = __builtin_popcount(h & HIM[tid]) == 1;
-      s_hbase[tid] = (anti && flip[HIK[tid]]) ? rowbase(h ^ HIM[tid]) : 0xffffffffu;
-    } else if (tid >= 64 && tid < 64 + NH) {
+    if (tid < 64) {
+      for (int b = tid; b < NHIGH; b += 64) {
+        const bool anti = __builtin_popcount(h & HIM[b]) == 1;
+        s_hbase[b] = (anti && flip[HIK[b]]) ? rowbase(h ^ HIM[b]) : kNone;
+      }
+    } else if (tid < 64 + NH) {
       s_cbase[tid - 64] = rowbase(h ^ (1u << (tid - 64)));
     } else if (tid == 192) {
       s_B = rowbase(h);
@@ -146,22 +170,22 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     if (tid == 0) {
       int n = 0;
       for (int b = 0; b < NHIGH; ++b)
-        if (s_hbase[b] != 0xffffffffu) { s_hbase[n] = s_hbase[b]; s_hcoef[n] = jh_[HIK[b]]; ++n; }
+        if (s_hbase[b] != kNone) { s_hbase[n] = s_hbase[b]; s_hcoef[n] = jh_[HIK[b]]; ++n; }
       s_nact = n;
     }
     __syncthreads();
-    const unsigned int B = __builtin_amdgcn_readfirstlane(s_B);
+    const BT B = rfl(s_B);
     const int nact = __builtin_amdgcn_readfirstlane(s_nact);
     const int cnt0 = __builtin_amdgcn_readfirstlane(s_cnt0), cnt1 = __builtin_amdgcn_readfirstlane(s_cnt1);
-    unsigned int cb[NH + 1];
+    BT cb[NH + 1];
 #pragma unroll
-    for (int q = 0; q < NH; ++q) cb[q] = __builtin_amdgcn_readfirstlane(s_cbase[q]);
+    for (int q = 0; q < NH; ++q) cb[q] = rfl(s_cbase[q]);
     const unsigned short* Uk = a.U + a.offU[kl];
 #pragma unroll 1
     for (int k = 0; k < kPerLane; ++k) {
       const int i = i0 + tid + k * kThreads;
       if (i >= R) break;
-      const unsigned int r = B + (unsigned int)i;
+      const BT r = B + (BT)i;
       double acc = 0.0;
       // high bonds: row h ^ m, same entry; loads issued in batches of 8
       for (int b0 = 0; b0 < nact; b0 += 8) {
@@ -169,7 +193,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           v[u] = 0.0;
-          if (b0 + u < nact) v[u] = xv[__builtin_amdgcn_readfirstlane(s_hbase[b0 + u]) + (unsigned int)i];
+          if (b0 + u < nact) v[u] = xv[rfl(s_hbase[b0 + u]) + (BT)i];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -205,7 +229,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
             } else {
               i2 = t0 + s_RL1[k0 * W1 + (l2 >> 8)];
             }
-            const unsigned int idx = on ? cb[hp] + i2 : r;
+            const BT idx = on ? cb[hp] + (BT)i2 : r;
             v[u] = xv[idx];
           }
         }
@@ -241,7 +265,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
               const unsigned int w0 = l2 & 255u;
               i2 = s_RL0[w0] + s_RL1[__builtin_popcount(w0) * W1 + (l2 >> 8)];
             }
-            const unsigned int idx = on ? B + i2 : r;
+            const BT idx = on ? B + (BT)i2 : r;
             v[u] = xv[idx];
           }
         }
@@ -328,6 +352,7 @@ struct JitArgs {
   const uint16_t* RL0;
   const uint16_t* RL1;
   const uint32_t* binom;
+  const uint64_t* binom64;
   const uint32_t* tasks;
   int ntasks;
   int offU[17];
@@ -396,7 +421,8 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
       }
     }
   std::ostringstream o;
-  o << "// generated by liblsqed (hv_jit.cpp)\n#define BATCH " << (batch ? 1 : 0) << "\n";
+  o << "// generated by liblsqed (hv_jit.cpp)\n#define BATCH " << (batch ? 1 : 0) << "\n#define WIDE "
+    << (N > 32 ? 1 : 0) << "\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
@@ -459,15 +485,7 @@ static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch
   return m;
 }
 
-// Split-plan device tables are shared with hv_split.hip; fetched through this
-// accessor to keep SplitPlan private there.
-struct SplitTables {
-  const uint16_t *U, *RL0, *RL1;
-  const uint32_t *binom, *tasks;
-  int ntasks;
-  const int* offU;
-};
-int split_tables(lsqed_ham* h, SplitTables& t);
+
 
 bool jit_enabled() {
   const char* e = getenv("LSQED_JIT");
@@ -509,6 +527,7 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
   a.RL0 = t.RL0;
   a.RL1 = t.RL1;
   a.binom = t.binom;
+  a.binom64 = t.binom64;
   a.tasks = t.tasks;
   a.ntasks = t.ntasks;
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
@@ -562,6 +581,7 @@ int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, in
   a.base.RL0 = t.RL0;
   a.base.RL1 = t.RL1;
   a.base.binom = t.binom;
+  a.base.binom64 = t.binom64;
   a.base.tasks = t.tasks;
   a.base.ntasks = t.ntasks;
   for (int k = 0; k < 17; ++k) a.base.offU[k] = t.offU[k];

@@ -40,7 +40,7 @@ struct SplitParams {
   const uint16_t* RL0;    // 256: rank contribution of the low byte
   const uint16_t* RL1;    // 9 x W1: contribution of bits 8..L-1, by popcount of the low byte
   const uint32_t* binom;  // C(n, k), n <= N, k <= nup: [(n)*(nup+1) + k]
-  const uint32_t* tasks;  // h | chunk << 16
+  const uint32_t* tasks;  // h | chunk << 24
   int N, nup, L, NH, ntasks, w1;  // w1 = 1 << max(L-8, 0)
   int nlow, ncross, nhigh;
   int offU[17];
@@ -58,6 +58,8 @@ struct SplitPlan {
   uint16_t* d_RL0 = nullptr;
   uint16_t* d_RL1 = nullptr;
   uint32_t* d_binom = nullptr;
+  uint64_t* d_binom64 = nullptr;  // the same binomials in 64 bits (N > 32)
+  bool classes_ok = false;        // the unspecialised kernel can run (N <= 32, classes fit)
   uint32_t* d_tasks = nullptr;
   int ntasks = 0;
   size_t binom_n = 0;
@@ -113,8 +115,8 @@ __global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, 
 
   for (int64_t t = ta; t < tb; ++t) {
     const uint32_t task = p.tasks[t];
-    const uint32_t h = task & 0xffffu;
-    const int chunk = (int)(task >> 16);
+    const uint32_t h = task & 0xffffffu;
+    const int chunk = (int)(task >> 24);
     const int kh = __builtin_popcount(h);
     const int kl = p.nup - kh;
     const int R = (int)C(p.L, kl);
@@ -276,6 +278,7 @@ void split_plan_destroy(lsqed_ham* h) {
   (void)hipFree(sp->d_RL0);
   (void)hipFree(sp->d_RL1);
   (void)hipFree(sp->d_binom);
+  (void)hipFree(sp->d_binom64);
   (void)hipFree(sp->d_tasks);
   delete sp;
   h->split = nullptr;
@@ -283,7 +286,7 @@ void split_plan_destroy(lsqed_ham* h) {
 
 int split_plan_create(lsqed_ham* h) {
   if (h->split) return LSQED_OK;
-  if (!split_supported(h)) return fail(LSQED_ERR_ARG, "row split needs N <= 32");
+  if (!split_plan_supported(h)) return fail(LSQED_ERR_ARG, "row-split tables need 2 <= N <= 40");
   const int N = h->N, nup = h->nup;
   const int L = std::min(16, N), NH = N - L;
   SplitPlan* sp = new SplitPlan;
@@ -319,26 +322,31 @@ int split_plan_create(lsqed_ham* h) {
       RL1[(size_t)k * p.w1 + v] = (uint16_t)r;
     }
   std::vector<uint32_t> bin((size_t)(N + 1) * (nup + 1));
+  std::vector<uint64_t> bin64((size_t)(N + 1) * (nup + 1));
   for (int n = 0; n <= N; ++n)
-    for (int k = 0; k <= nup; ++k) bin[(size_t)n * (nup + 1) + k] = (uint32_t)binom_u64(n, k);
-  // bond classes
-  for (int kind = 0; kind < 2; ++kind) {
+    for (int k = 0; k <= nup; ++k) {
+      bin64[(size_t)n * (nup + 1) + k] = binom_u64(n, k);
+      bin[(size_t)n * (nup + 1) + k] = (uint32_t)bin64[(size_t)n * (nup + 1) + k];  // N <= 32: exact
+    }
+  // bond classes of the unspecialised kernel (N <= 32, at most kMaxClass
+  // bonds per class; the specialised kernel emits its own classes)
+  sp->classes_ok = N <= 32;
+  for (int kind = 0; kind < 2 && sp->classes_ok; ++kind) {
     p.nbk[kind] = (int)h->bi[kind].size();
-    for (size_t b = 0; b < h->bi[kind].size(); ++b) {
+    for (size_t b = 0; b < h->bi[kind].size() && sp->classes_ok; ++b) {
       int a = h->bi[kind][b], c = h->bj[kind][b];
       int lo = std::min(a, c), hi = std::max(a, c);
       if (lo == hi) continue;  // a self bond is always parallel: diagonal only (counted via nbk)
       if (hi < L) {
         const int type = hi < 8 ? 0 : (lo >= 8 ? 1 : 2);
+        if (p.nlow == kMaxClass) { sp->classes_ok = false; break; }
         p.low[p.nlow++] = ((1u << lo) | (1u << hi)) | ((uint32_t)kind << 16) | ((uint32_t)type << 17);
       } else if (lo >= L) {
+        if (p.nhigh == kMaxClass) { sp->classes_ok = false; break; }
         p.high[p.nhigh++] = ((1u << (lo - L)) | (1u << (hi - L))) | ((uint32_t)kind << 16);
       } else {
+        if (p.ncross == kMaxClass) { sp->classes_ok = false; break; }
         p.cross[p.ncross++] = (uint32_t)lo | ((uint32_t)(hi - L) << 4) | ((uint32_t)kind << 8);
-      }
-      if (p.nlow > kMaxClass || p.nhigh > kMaxClass || p.ncross > kMaxClass) {
-        delete sp;
-        return fail(LSQED_ERR_ARG, "too many bonds of one class for the row split");
       }
     }
   }
@@ -348,7 +356,7 @@ int split_plan_create(lsqed_ham* h) {
     const int kl = nup - count_bits(hh);
     if (kl < 0 || kl > L) continue;
     const int64_t R = (int64_t)binom_u64(L, kl);
-    for (int64_t c = 0; c * kSplitChunk < R; ++c) tasks.push_back(hh | ((uint32_t)c << 16));
+    for (int64_t c = 0; c * kSplitChunk < R; ++c) tasks.push_back(hh | ((uint32_t)c << 24));
   }
   sp->ntasks = (int)tasks.size();
   p.ntasks = sp->ntasks;
@@ -358,6 +366,7 @@ int split_plan_create(lsqed_ham* h) {
   };
   if (!up((void**)&sp->d_U, U.data(), U.size() * 2) || !up((void**)&sp->d_RL0, RL0.data(), RL0.size() * 2) ||
       !up((void**)&sp->d_RL1, RL1.data(), RL1.size() * 2) || !up((void**)&sp->d_binom, bin.data(), bin.size() * 4) ||
+      !up((void**)&sp->d_binom64, bin64.data(), bin64.size() * 8) ||
       !up((void**)&sp->d_tasks, tasks.data(), tasks.size() * 4)) {
     h->split = sp;
     split_plan_destroy(h);
@@ -372,13 +381,6 @@ int split_plan_create(lsqed_ham* h) {
   return LSQED_OK;
 }
 
-struct SplitTables {
-  const uint16_t *U, *RL0, *RL1;
-  const uint32_t *binom, *tasks;
-  int ntasks;
-  const int* offU;
-};
-
 int split_tables(lsqed_ham* h, SplitTables& t) {
   int rc = split_plan_create(h);
   if (rc) return rc;
@@ -387,6 +389,7 @@ int split_tables(lsqed_ham* h, SplitTables& t) {
   t.RL0 = sp->d_RL0;
   t.RL1 = sp->d_RL1;
   t.binom = sp->d_binom;
+  t.binom64 = sp->d_binom64;
   t.tasks = sp->d_tasks;
   t.ntasks = sp->ntasks;
   t.offU = sp->p.offU;
@@ -394,14 +397,17 @@ int split_tables(lsqed_ham* h, SplitTables& t) {
 }
 
 // The specialised kernel handles up to 64 high bonds (one prologue lane each).
+bool split_plan_supported(const lsqed_ham* h) { return h->N >= 2 && h->N <= 40; }
+
 bool split_jit_fits(const lsqed_ham* h) {
-  if (!split_supported(h)) return false;
+  if (!split_plan_supported(h)) return false;
+  if ((int)h->bi[0].size() + (int)h->bi[1].size() > 2 * kMaxBonds) return false;
   const int L = std::min(16, h->N);
   int nhigh = 0;
   for (int k = 0; k < 2; ++k)
     for (size_t b = 0; b < h->bi[k].size(); ++b)
       if (std::min(h->bi[k][b], h->bj[k][b]) >= L && h->bi[k][b] != h->bj[k][b]) ++nhigh;
-  return nhigh <= 64;
+  return nhigh <= 256;
 }
 
 int split_grid(const lsqed_ham* h) {
@@ -422,6 +428,8 @@ int launch_hv_split(lsqed_ham* h, const double* x, double* y, double* dot_part, 
   int rc = split_plan_create(h);
   if (rc) return rc;
   SplitPlan* sp = (SplitPlan*)h->split;
+  if (!sp->classes_ok)
+    return fail(LSQED_ERR_ARG, "the unspecialised row split needs N <= 32 and at most 64 bonds per class");
   SplitParams p = sp->p;
   const RowParams rp = make_row_params(h);
   p.fields = h->d_fields;

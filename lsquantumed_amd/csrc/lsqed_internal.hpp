@@ -122,8 +122,19 @@ int ensure_scratch(lsqed_ham* h, size_t doubles);
 int hv_launch_grid(lsqed_ham* h);
 int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, hipStream_t s);
 
-// Row-split kernel (hv_split.hip).
+// Row-split kernel (hv_split.hip).  The unspecialised kernel handles
+// N <= 32 (split_supported); the tables (split plan) and the specialised
+// kernel go to N <= 40 with 64-bit row bases above 32.
+struct SplitTables {
+  const uint16_t *U, *RL0, *RL1;
+  const uint32_t *binom, *tasks;
+  const uint64_t* binom64;
+  int ntasks;
+  const int* offU;
+};
 bool split_supported(const lsqed_ham* h);
+bool split_plan_supported(const lsqed_ham* h);
+int split_tables(lsqed_ham* h, SplitTables& t);
 int split_plan_create(lsqed_ham* h);
 void split_plan_destroy(lsqed_ham* h);
 int split_grid(const lsqed_ham* h);

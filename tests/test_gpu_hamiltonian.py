@@ -239,12 +239,16 @@ def formula_vector(O, torch, D, chunk=1 << 27):
     return x
 
 
-def test_hv_6x6_rows_formula_vector(O, torch):
+@pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
+def test_hv_6x6_rows_formula_vector(O, torch, kernel):
     """6x6 J1-J2 (N = 36, D = 9,075,135,300; 72.6 GB per vector): rows of
-    H x sampled across the sector (64-bit ranks) against the oracle, for x
-    given by a formula of the rank that the oracle evaluates itself."""
+    H x sampled across the sector (64-bit ranks, the 2^32 boundary included)
+    against the oracle, for x given by a formula of the rank that the oracle
+    evaluates itself; the generic kernel and the specialised row split with
+    64-bit row bases."""
     import time
     h = make("square", 36, 0, 1.0, 1.0, 0.5, 0.5, 6, 6)
+    h.set_hv_kernel(kernel)
     D = h.hamilt_size()
     assert D == 9075135300
     say("hv 6x6: kernel", h.hv_kernel())
