@@ -174,6 +174,37 @@ def disorder_batch(reps: int = 3):
     }
 
 
+def krylov_schur_4x6():
+    """Krylov-Schur nev=10, ncv=32, tol 1e-9 on the periodic 4x6 square
+    lattice, J1=D1=1, J2=D2=0.5, Sz=0 (D = 2,704,156): wall time of
+    SolverInit + solve after one untimed solve, with SLEPc's stopping rule and
+    with the multiplicity check (which finds both copies of the degenerate
+    levels among the 10 lowest)."""
+    import torch
+
+    import lsquantumed_amd as Q
+
+    env = Q.Environment(24)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.square2D(env, 4, 6), 1.0, 1.0, 0.5, 0.5)
+    h.build_diag(env)
+    h.build_off_diag(env)
+    out = {"workload": "Krylov-Schur nev=10 ncv=32 tol 1e-9, periodic square 4x6 (N=24), J1=D1=1, J2=D2=0.5, "
+                       "Sz=0, D=2704156"}
+    for name, rounds in (("slepc_rule", 0), ("multiplicity_check", 3)):
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            eps = Q.Solver.SolverInit(h, 10, 32, -2, tol=1e-9, multiplicity_rounds=rounds)
+            nconv = Q.Solver.solve(env, eps)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            vals = [Q.Solver.solution(eps, n)[0] for n in range(min(nconv, 10))]
+            its = eps.iterations()
+            Q.Solver.SolverClean(eps)
+        out[name] = {"solve_s": round(dt, 4), "restarts": its, "nconv": nconv, "eigenvalues": vals}
+    return out
+
+
 def lanczos_6x6(maxit: int = 600, tol: float = 1e-9):
     """Ground state of the periodic 6x6 square lattice, J1=D1=1, J2=D2=0.5,
     Sz=0 (N=36, D = 9,075,135,300, 72.6 GB per vector) with the three-vector
@@ -333,6 +364,7 @@ def main():
     del q, p, y
     torch.cuda.empty_cache()
     batch = None if args.no_batch else disorder_batch()
+    ks = None if args.no_batch else krylov_schur_4x6()
     cpu = None if args.no_cpu else cpu_baseline((N, N // 2, J1, D1, J2, D2, lx, ly, B))
     out = {
         "metric": METRIC,
@@ -358,6 +390,7 @@ def main():
                      "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name, lx, ly)},
         "cpu_baseline": cpu,
         "batched_disorder": batch,
+        "krylov_schur_4x6": ks,
         "alpha_last": alphas[-1],
     }
     print(json.dumps(out))

@@ -142,6 +142,14 @@ typedef struct {
   double tol;
   int maxit;
   uint64_t seed;  /* start vector: splitmix64 hash of (seed, row), uniform in [-1, 1) */
+  /* Multiplicity check (no counterpart in SLEPc): once nev values have
+   * converged they are locked and the iteration continues from a random
+   * vector orthogonal to them until one more value converges, so that copies
+   * of a degenerate level that the single start vector did not reach are
+   * found; repeated while it changes the nev lowest, at most this many
+   * rounds.  <= 0: off -- SLEPc's stopping rule, which the reference's tests
+   * pin (tests/unit_tests/solver: one copy per level of a diagonal H). */
+  int multiplicity_rounds;
 } lsqed_eps_opts;
 
 typedef struct lsqed_eps lsqed_eps;

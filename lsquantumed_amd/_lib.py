@@ -33,7 +33,7 @@ class Params(C.Structure):
 
 class EpsOpts(C.Structure):
     _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("mpd", C.c_int), ("tol", C.c_double),
-                ("maxit", C.c_int), ("seed", C.c_uint64)]
+                ("maxit", C.c_int), ("seed", C.c_uint64), ("multiplicity_rounds", C.c_int)]
 
 
 class BatchOpts(C.Structure):

@@ -299,6 +299,7 @@ struct lsqed_eps {
   int nev = 1, ncv = 0, mpd = 0, maxit = 0;
   double tol = 1e-8;
   uint64_t seed = 1;
+  int verify_rounds = 0;   // multiplicity check rounds (0: off, SLEPc's stopping rule)
   double* V = nullptr;     // (ncv + 1) columns of D
   double* part = nullptr;  // kMaxNcv * kRedBlocks partials
   double* dsm = nullptr;   // small device buffer: coefficients / S matrix
@@ -438,6 +439,7 @@ int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out) {
   e->tol = d.tol > 0 ? d.tol : 1e-8;
   e->maxit = d.maxit > 0 ? d.maxit : (int)std::max<int64_t>(100, 2 * D / ncv);
   e->seed = d.seed ? d.seed : 1;
+  e->verify_rounds = std::max(0, d.multiplicity_rounds);
   const size_t bytes = (size_t)(ncv + 1) * D * sizeof(double);
   if (hipMalloc((void**)&e->V, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -486,6 +488,16 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
   e->nconv = 0;
   e->evals.clear();
   uint64_t reseed = e->seed;
+  // Multiplicity check: a single start vector spans one direction per
+  // distinct eigenvalue; further copies of a degenerate level enter only
+  // through rounding and can be missed when the wanted values converge
+  // first.  Once nev values have converged, they are locked and the
+  // iteration continues from a random vector orthogonal to them until one
+  // more value has converged; if the lowest nev changed (a missed copy
+  // surfaced), the check is repeated.
+  int rounds_left = e->verify_rounds;
+  int target = e->nev;
+  std::vector<double> prev;
   while (true) {
     int meff = m;
     bool breakdown = false;
@@ -520,8 +532,27 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
       res[i] = breakdown ? 0.0 : std::fabs(beta[meff - 1] * S[(size_t)(meff - 1) * meff + i]);
     int nconv = 0;
     while (nconv < meff && res[nconv] <= e->tol * std::max(std::fabs(w[nconv]), 1e-300)) ++nconv;
-    const bool enough = nconv >= e->nev;
+    bool enough = nconv >= target;
     const bool full = breakdown && meff >= m;
+    if (enough && rounds_left > 0 && !full && meff < D && e->nev + 2 <= m && (int64_t)e->nev + 1 < D &&
+        e->its < e->maxit) {
+      bool changed = prev.empty();
+      for (size_t i = 0; i < prev.size() && !changed; ++i)
+        changed = std::fabs(w[i] - prev[i]) > std::max(100.0 * e->tol, 1e-10) * std::max(std::fabs(prev[i]), 1.0);
+      if (changed) {
+        prev.assign(w.begin(), w.begin() + e->nev);
+        --rounds_left;
+        if ((rc = restart_transform(c, meff, e->nev, S))) return rc;
+        k = e->nev;
+        arrow.assign(k, 0.0);
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int i = 0; i < k; ++i) T[(size_t)i * m + i] = w[i];
+        if ((rc = fresh_vector(c, k, ++reseed))) return rc;
+        target = e->nev + 1;
+        continue;
+      }
+    }
+    if (!enough && nconv >= e->nev && e->its >= e->maxit) enough = true;
     if (enough || e->its >= e->maxit || full) {
       int kfin = enough ? nconv : std::max(nconv, std::min(e->nev, meff));
       kfin = std::min(kfin, meff);

@@ -24,10 +24,11 @@ PETSC_DECIDE = -1
 class EPS:
     """Handle of one eigensolve (the reference's `EPS` object)."""
 
-    def __init__(self, h, nev, ncv=PETSC_DEFAULT, mpd=PETSC_DEFAULT, tol=None, maxit=None, seed=1):
+    def __init__(self, h, nev, ncv=PETSC_DEFAULT, mpd=PETSC_DEFAULT, tol=None, maxit=None, seed=1,
+                 multiplicity_rounds=0):
         self.h = h
         o = _lib.EpsOpts(int(nev), int(ncv) if ncv and ncv > 0 else 0, int(mpd) if mpd and mpd > 0 else 0,
-                         float(tol) if tol else 0.0, int(maxit) if maxit else 0, int(seed))
+                         float(tol) if tol else 0.0, int(maxit) if maxit else 0, int(seed), int(multiplicity_rounds))
         e = C.c_void_p()
         check(lib().lsqed_eps_create(h.hamilt, C.byref(o), C.byref(e)))
         self.handle = e
@@ -50,8 +51,12 @@ class EPS:
 
 class Solver:
     @staticmethod
-    def SolverInit(h, nev, ncv=PETSC_DEFAULT, mpd=PETSC_DEFAULT, tol=None, maxit=None, seed=1) -> EPS:
-        return EPS(h, nev, ncv, mpd, tol, maxit, seed)
+    def SolverInit(h, nev, ncv=PETSC_DEFAULT, mpd=PETSC_DEFAULT, tol=None, maxit=None, seed=1,
+                   multiplicity_rounds=0) -> EPS:
+        """multiplicity_rounds > 0: after convergence, look for copies of
+        degenerate levels that the start vector missed (at most that many
+        rounds; include/lsqed.h); 0 = SLEPc's stopping rule."""
+        return EPS(h, nev, ncv, mpd, tol, maxit, seed, multiplicity_rounds)
 
     @staticmethod
     def solve(env, eps: EPS, stream=None) -> int:

@@ -178,3 +178,68 @@ def test_solver_errors(torch):
     eps = Q.Solver.SolverInit(h, 1)
     with pytest.raises(Q.LsqedError):
         Q.Solver.solution(eps, 0)  # before solve
+
+
+def test_square4x6_j1j2_nev10(O, torch):
+    """Periodic 4x6, J1=D1=1, J2=D2=0.5, Sz=0 (D = 2,704,156): Krylov-Schur
+    nev=10, ncv=32, tol 1e-9 against the oracle's spectrum (committed fixture,
+    tests/golden/make_config2.py).  With SLEPc's stopping rule every returned
+    value is an eigenvalue (1e-9 relative), the levels come in order and none
+    is skipped (copies of a degenerate level may be missing, as in SLEPc).
+    With the multiplicity check: the 10 lowest with multiplicity to 1e-9; the
+    ground state's <S_i.S_j> for all pairs, the squared staggered
+    magnetisation, and the DSF S^z weights at q = (pi, pi) of the 10 states
+    summed per degenerate multiplet, to 1e-8."""
+    import json
+    import os
+    import time
+
+    import lsquantumed_amd as Q
+    from lsquantumed_amd import correlations
+    with open(os.path.join(os.path.dirname(__file__), "golden", "config2_square4x6_j1j2.json")) as f:
+        fx = json.load(f)
+    env = Q.Environment(24)
+    lat = Q.square2D(env, 4, 6)
+    h = make("square", 24, 0, 1.0, 1.0, 0.5, 0.5, 4, 6)
+    D = h.hamilt_size()
+    assert D == fx["D"]
+    ref = np.array(fx["eigenvalues"])
+    levels = [ref[g[0]] for g in O.multiplets(ref)]
+    eps = Q.Solver.SolverInit(h, 10, 32, -2, tol=1e-9)
+    nconv = Q.Solver.solve(env, eps)
+    assert nconv >= 10
+    got = [Q.Solver.solution(eps, n)[0] for n in range(10)]
+    Q.Solver.SolverClean(eps)
+    say("SLEPc rule:", np.round(got, 9))
+    glev = [got[g[0]] for g in O.multiplets(np.array(got))]
+    for a, b in zip(glev, levels):
+        assert abs(a - b) <= 1e-9 * abs(b), (glev, levels)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eps = Q.Solver.SolverInit(h, 10, 32, -2, tol=1e-9, multiplicity_rounds=3)
+    nconv = Q.Solver.solve(env, eps)
+    torch.cuda.synchronize()
+    say("4x6 nev=10 ncv=32: nconv", nconv, "iterations", eps.iterations(), "s", round(time.perf_counter() - t, 3))
+    assert nconv >= 10
+    ev = torch.empty((10, D), dtype=torch.float64, device="cuda")
+    vals, errs = [], []
+    for n in range(10):
+        lam, err = Q.Solver.solution(eps, n, ev[n])
+        vals.append(lam)
+        errs.append(err)
+    Q.Solver.SolverClean(eps)
+    rel = np.abs(np.array(vals) - ref[:10]) / np.abs(ref[:10])
+    say("eigenvalue rel err", rel.max(), "residuals", max(errs))
+    assert rel.max() <= 1e-9 and max(errs) <= 1e-9
+    psi0 = ev[0]
+    c = correlations(h, psi0)
+    d_sisj = np.abs(c["SiSj"] - np.array(fx["sisj"])).max()
+    m2 = Q.Phys.SquaredSubLattMagAF(env, h, Q.AF(lat), psi0)
+    amp = Q.Phys.dsf_amplitudes(h, h, psi0, ev)
+    coords = np.array([[i // 6, i % 6] for i in range(24)], float)
+    d_dsf = 0.0
+    for grp in fx["dsf_pipi"]:
+        W = sum(O.dsf_weight(amp[n], (np.pi, np.pi), coords) for n in grp["states"])
+        d_dsf = max(d_dsf, abs(W - grp["weight"]))
+    say("max |d SiSj|", d_sisj, "|d m2|", abs(m2 - fx["m2_af"]), "max |d W(pi,pi)|", d_dsf)
+    assert d_sisj <= 1e-8 and abs(m2 - fx["m2_af"]) <= 1e-8 and d_dsf <= 1e-8
