@@ -138,6 +138,64 @@ __global__ __launch_bounds__(kThreads) void k_update(const double* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// One pass of classical Gram-Schmidt over rows, m <= MAXM basis columns
+// held in registers (all m loads of a row are issued before any is used):
+//   MODE 0: part[j] += V_j . w                          (projection)
+//   MODE 1: w <- w - V h;  part[j] += V_j . w; part[m] += ||w||^2
+//           (the first update fused with the second projection of CGS2)
+//   MODE 2: w <- w - V h;  part[0] += ||w||^2           (final update)
+// part is column-major over blocks: part[j * gridDim.x + block].
+template <int MAXM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_cgs_pass(const double* __restrict__ V, int64_t ld, int m,
+                                                       const double* __restrict__ hcoef, double* __restrict__ w,
+                                                       int64_t n, double* __restrict__ part) {
+  __shared__ double hs[MAXM];
+  __shared__ double red[kThreads / 64][MAXM + 1];
+  if (MODE != 0)
+    for (int j = threadIdx.x; j < MAXM; j += blockDim.x) hs[j] = j < m ? hcoef[j] : 0.0;
+  __syncthreads();
+  int64_t a, b;
+  block_range(n, a, b);
+  double acc[MAXM + 1];
+#pragma unroll
+  for (int j = 0; j <= MAXM; ++j) acc[j] = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+    double vin[MAXM];
+#pragma unroll
+    for (int j = 0; j < MAXM; ++j) vin[j] = j < m ? V[(int64_t)j * ld + r] : 0.0;
+    double wr = w[r];
+    if (MODE != 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < MAXM; ++j) s += vin[j] * hs[j];
+      wr -= s;
+      w[r] = wr;
+      acc[MAXM] += wr * wr;
+    }
+    if (MODE != 2) {
+#pragma unroll
+      for (int j = 0; j < MAXM; ++j) acc[j] += vin[j] * wr;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j <= MAXM; ++j) {
+    if ((MODE == 2 && j < MAXM) || (MODE == 0 && j == MAXM)) continue;
+    const double t = wsum(acc[j]);
+    if (lane == 0) red[wid][j] = t;
+  }
+  __syncthreads();
+  const int G = gridDim.x;
+  for (int j = threadIdx.x; j <= MAXM; j += blockDim.x) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][j];
+    if (MODE == 0 && j < m) part[(int64_t)j * G + blockIdx.x] = t;
+    if (MODE == 1 && j < m) part[(int64_t)j * G + blockIdx.x] = t;
+    if (MODE == 1 && j == MAXM) part[(int64_t)m * G + blockIdx.x] = t;
+    if (MODE == 2 && j == MAXM) part[blockIdx.x] = t;
+  }
+}
+
 __global__ void k_scale(double* __restrict__ v, int64_t n, double a) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
     v[r] *= a;
@@ -363,6 +421,50 @@ int scale(Ctx& c, double* x, double a) {
 // rounding noise is mostly outside span(V), survives the test and becomes the
 // next basis vector -- which is how degenerate copies of an eigenvalue enter
 // the Krylov space.
+template <int MAXM, int MODE>
+void launch_cgs_pass(Ctx& c, int G, int m, const double* hdev, double* w) {
+  hipLaunchKernelGGL((k_cgs_pass<MAXM, MODE>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, m, hdev, w, c.D,
+                     c.e->part);
+}
+
+// CGS2 in three passes over V for m <= 32 (projection; first update fused
+// with the second projection; second update), each with all column loads of
+// a row in flight.
+int cgs2_fused(Ctx& c, int m, double* w, std::vector<double>& h1, std::vector<double>& h2, double& nrm1,
+               double& nrm) {
+  const int G = red_blocks(c.D);
+  double* hdev = c.e->dsm + kMaxNcv * kMaxNcv;  // coefficient staging (2 kMaxNcv doubles)
+  std::vector<double> out(m + 1);
+  int rc;
+  auto pass = [&](int mode, const std::vector<double>* hc) -> int {
+    if (hc) LSQED_HIP(hipMemcpyAsync(hdev, hc->data(), m * sizeof(double), hipMemcpyHostToDevice, c.s));
+    if (m <= 16) {
+      if (mode == 0) launch_cgs_pass<16, 0>(c, G, m, hdev, w);
+      if (mode == 1) launch_cgs_pass<16, 1>(c, G, m, hdev, w);
+      if (mode == 2) launch_cgs_pass<16, 2>(c, G, m, hdev, w);
+    } else if (m <= 24) {
+      if (mode == 0) launch_cgs_pass<24, 0>(c, G, m, hdev, w);
+      if (mode == 1) launch_cgs_pass<24, 1>(c, G, m, hdev, w);
+      if (mode == 2) launch_cgs_pass<24, 2>(c, G, m, hdev, w);
+    } else {
+      if (mode == 0) launch_cgs_pass<32, 0>(c, G, m, hdev, w);
+      if (mode == 1) launch_cgs_pass<32, 1>(c, G, m, hdev, w);
+      if (mode == 2) launch_cgs_pass<32, 2>(c, G, m, hdev, w);
+    }
+    LSQED_HIP(hipGetLastError());
+    const int ncol = mode == 0 ? m : (mode == 1 ? m + 1 : 1);
+    return finish_reduce(c.e->h, c.e->part, G, ncol, c.e->dsm, out.data(), c.s);
+  };
+  if ((rc = pass(0, nullptr))) return rc;
+  h1.assign(out.begin(), out.begin() + m);
+  if ((rc = pass(1, &h1))) return rc;
+  h2.assign(out.begin(), out.begin() + m);
+  nrm1 = std::sqrt(out[m]);
+  if ((rc = pass(2, &h2))) return rc;
+  nrm = std::sqrt(out[0]);
+  return LSQED_OK;
+}
+
 int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* lindep = nullptr) {
   std::vector<double> h1, h2;
   int rc;
@@ -373,10 +475,14 @@ int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* li
     return rc;
   }
   double nrm1 = 0.0;
-  if ((rc = multidot(c, m, w, h1))) return rc;
-  if ((rc = update(c, m, w, h1, nrm1))) return rc;
-  if ((rc = multidot(c, m, w, h2))) return rc;
-  if ((rc = update(c, m, w, h2, nrm))) return rc;
+  if (m <= 32) {
+    if ((rc = cgs2_fused(c, m, w, h1, h2, nrm1, nrm))) return rc;
+  } else {
+    if ((rc = multidot(c, m, w, h1))) return rc;
+    if ((rc = update(c, m, w, h1, nrm1))) return rc;
+    if ((rc = multidot(c, m, w, h2))) return rc;
+    if ((rc = update(c, m, w, h2, nrm))) return rc;
+  }
   h.resize(m);
   for (int j = 0; j < m; ++j) h[j] = h1[j] + h2[j];
   if (lindep) *lindep = !(nrm > 0.0 && nrm >= 0.7071067811865476 * nrm1);
