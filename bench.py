@@ -174,6 +174,39 @@ def disorder_batch(reps: int = 3):
     }
 
 
+def stream_ceiling(q, p, y, reps: int = 10):
+    """Measured streaming ceilings on the same buffers (HIP events): the
+    library's read-only stream (8 B per element) and STREAM triad (24 B per
+    element), and torch's elementwise add as a second triad."""
+    import ctypes as C
+
+    import torch
+
+    from lsquantumed_amd._lib import check, lib
+
+    n = q.numel() - (q.numel() & 1)
+    part = torch.empty(2048, dtype=torch.float64, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(f):
+        f()
+        ev0.record()
+        for _ in range(reps):
+            f()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    ms_r = timed(lambda: check(lib().lsqed_stream_probe(1, None, C.c_void_p(q.data_ptr()), None, n,
+                                                         C.c_void_p(part.data_ptr()), st)))
+    ms_t = timed(lambda: check(lib().lsqed_stream_probe(0, C.c_void_p(y.data_ptr()), C.c_void_p(q.data_ptr()),
+                                                         C.c_void_p(p.data_ptr()), n, None, st)))
+    ms_tt = timed(lambda: torch.add(q[:n], p[:n], alpha=0.5, out=y[:n]))
+    return {"read_gbs": round(8 * n / ms_r / 1e6, 1), "triad_gbs": round(24 * n / ms_t / 1e6, 1),
+            "torch_triad_gbs": round(24 * n / ms_tt / 1e6, 1), "elements": n}
+
+
 def krylov_schur_4x6():
     """Krylov-Schur nev=10, ncv=32, tol 1e-9 on the periodic 4x6 square
     lattice, J1=D1=1, J2=D2=0.5, Sz=0 (D = 2,704,156): wall time of
@@ -203,6 +236,44 @@ def krylov_schur_4x6():
             Q.Solver.SolverClean(eps)
         out[name] = {"solve_s": round(dt, 4), "restarts": its, "nconv": nconv, "eigenvalues": vals}
     return out
+
+
+def krylov_schur_4x8(tol: float = 1e-9):
+    """Krylov-Schur nev=4, ncv=20 on the periodic 4x8 square lattice,
+    J1=D1=1, J2=D2=0.5, Sz=0 (D = 601,080,390): the basis is 21 vectors of
+    4.81 GB.  Reports the eigenvalues, EPS_ERROR_RELATIVE of each, the
+    restarts and the wall time, and the ground energy of the three-vector
+    Lanczos on the same problem as a cross-check."""
+    import torch
+
+    import lsquantumed_amd as Q
+
+    env = Q.Environment(32)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.square2D(env, 4, 8), 1.0, 1.0, 0.5, 0.5)
+    h.build_diag(env)
+    h.build_off_diag(env)
+    print("4x8 Krylov-Schur nev=4 ncv=20: start", flush=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eps = Q.Solver.SolverInit(h, 4, 20, -2, tol=tol)
+    nconv = Q.Solver.solve(env, eps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    sol = [Q.Solver.solution(eps, n) for n in range(min(nconv, 4))]
+    its = eps.iterations()
+    Q.Solver.SolverClean(eps)
+    print(f"4x8 Krylov-Schur: {dt:.1f} s, {its} restarts, nconv {nconv}", flush=True)
+    del eps
+    torch.cuda.empty_cache()
+    t = time.perf_counter()
+    e0, steps = Q.lanczos_ground(h, maxit=1000, tol=1e-13)
+    dl = time.perf_counter() - t
+    return {"workload": f"Krylov-Schur nev=4 ncv=20 tol {tol:g}, periodic square 4x8 (N=32), J1=D1=1, J2=D2=0.5, "
+                        "Sz=0, D=601080390 (basis 21 x 4.81 GB)",
+            "solve_s": round(dt, 2), "restarts": its, "nconv": nconv,
+            "eigenvalues": [v for v, _ in sol], "relative_errors": [e for _, e in sol],
+            "lanczos3_E0": e0, "lanczos3_steps": steps, "lanczos3_s": round(dl, 2),
+            "E0_diff_krylov_schur_vs_lanczos3": abs(sol[0][0] - e0) if sol else None}
 
 
 def lanczos_6x6(maxit: int = 600, tol: float = 1e-9):
@@ -284,6 +355,8 @@ def main():
                     help="only the batched-disorder measurement (one JSON object; used for profiling)")
     ap.add_argument("--lanczos-6x6", action="store_true",
                     help="only the 6x6 (N=36) three-vector Lanczos ground state (218 GB; minutes)")
+    ap.add_argument("--krylov-4x8", action="store_true",
+                    help="only Krylov-Schur nev=4 ncv=20 on the 4x8 sector (101 GB basis; about a minute)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Sz-sector path here is built and measured on a single "
@@ -300,6 +373,9 @@ def main():
         return
     if args.lanczos_6x6:
         print(json.dumps({"lanczos_6x6": lanczos_6x6()}))
+        return
+    if args.krylov_4x8:
+        print(json.dumps({"krylov_schur_4x8": krylov_schur_4x8()}))
         return
     lx, ly = args.lx, args.ly
     N = lx * ly
@@ -361,6 +437,7 @@ def main():
     step_ms = ev0.elapsed_time(ev1) / args.steps
     steps_per_s = 1e3 / step_ms
 
+    stream = stream_ceiling(q, p, y)
     del q, p, y
     torch.cuda.empty_cache()
     batch = None if args.no_batch else disorder_batch()
@@ -387,7 +464,10 @@ def main():
         "lanczos_steps_per_s": round(steps_per_s, 3),
         "lanczos_wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(hv_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name, lx, ly)},
+                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name, lx, ly),
+                     "measured_read_stream_gbs": stream["read_gbs"],
+                     "measured_triad_gbs": max(stream["triad_gbs"], stream["torch_triad_gbs"])},
+        "stream": stream,
         "cpu_baseline": cpu,
         "batched_disorder": batch,
         "krylov_schur_4x6": ks,

@@ -265,6 +265,14 @@ int lsqed_batch_ground_state(lsqed_batch* b, int k, double* psi, lsqed_stream_t 
 int lsqed_batch_observables(lsqed_batch* b, double* out, lsqed_stream_t stream);
 int lsqed_batch_destroy(lsqed_batch* b);
 
+/* ----------------------------------------------------------- measurement -- */
+/* Streaming-bandwidth probes, no counterpart in the reference: the measured
+ * ceiling reported beside the nominal HBM peak.  kind 0: STREAM triad
+ * a = b + 0.5 c (24 B per element); kind 1: read-only stream of b (8 B per
+ * element; n even; part: 2048 doubles of block sums).  Device buffers. */
+int lsqed_stream_probe(int kind, double* a, const double* b, const double* c, int64_t n, double* part,
+                       lsqed_stream_t stream);
+
 /* ---------------------------------------------------------------- misc ---- */
 /* Tools::RandomDisorder (src/tools/random_disorder.cpp:33-59): the
  * reproducible seed list (mt19937_64(1) -> uniform_int(0,1000)) and the

@@ -82,6 +82,7 @@ SIGNATURES = {
     "lsqed_batch_observables": (_i32, [_vp, _vp, _vp]),
     "lsqed_batch_destroy": (_i32, [_vp]),
     "lsqed_batch_workspace_bytes": (_i64, [_i32, _P(_vp), _P(BatchOpts)]),
+    "lsqed_stream_probe": (_i32, [_i32, _vp, _vp, _vp, _i64, _vp, _vp]),
     "lsqed_rd_seeds": (_i32, [_i32, _vp]),
     "lsqed_rd_fields": (_i32, [_i64, _i32, _dbl, _dbl, _vp]),
     "lsqed_last_error": (C.c_char_p, []),

@@ -253,6 +253,27 @@ __global__ void k_resid(const double* __restrict__ y, const double* __restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// Streaming-bandwidth probes (measurement only): a read-only stream and a
+// STREAM triad, in the access pattern that measured fastest on gfx950
+// (16-byte loads, grid-stride, ~4 blocks per CU).
+__global__ __launch_bounds__(kThreads) void k_stream_read(const double* __restrict__ b, int64_t n2,
+                                                          double* __restrict__ part) {
+  const double2* B = reinterpret_cast<const double2*>(b);
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = B[i];
+    acc += x.x + x.y;
+  }
+  const double t = bsum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_stream_triad(double* __restrict__ a, const double* __restrict__ b,
+                                                           const double* __restrict__ c, int64_t n, double s) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = b[i] + s * c[i];
+}
+
 static int red_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + 255) / 256)); }
 static int stream_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256)); }
 
@@ -717,6 +738,23 @@ int lsqed_eps_solution(lsqed_eps* e, int i, double* lambda, double* x, double* e
     *error = std::sqrt(ss) / std::max(std::fabs(lam), 1e-300);
   }
   LSQED_HIP(hipStreamSynchronize(c.s));
+  return LSQED_OK;
+}
+
+int lsqed_stream_probe(int kind, double* a, const double* b, const double* c, int64_t n, double* part,
+                       lsqed_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n <= 0 || !b) return fail(LSQED_ERR_ARG, "empty stream");
+  if (kind == 0) {
+    if (!a || !c) return fail(LSQED_ERR_ARG, "triad needs a, b and c");
+    hipLaunchKernelGGL(k_stream_triad, dim3(1024), dim3(kThreads), 0, s, a, b, c, n, 0.5);
+  } else if (kind == 1) {
+    if (!part || (n & 1)) return fail(LSQED_ERR_ARG, "read stream needs an even n and 2048 partials");
+    hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(kThreads), 0, s, b, n / 2, part);
+  } else {
+    return fail(LSQED_ERR_ARG, "kind must be 0 (triad) or 1 (read)");
+  }
+  LSQED_HIP(hipGetLastError());
   return LSQED_OK;
 }
 
