@@ -11,5 +11,7 @@ from .solver import EPS, PETSC_DECIDE, PETSC_DEFAULT, Solver, lanczos_ground, la
 from .tools import RandomDisorder, fields_from_seed  # noqa: F401
 from .batch import BatchSolver, disorder_chains  # noqa: F401
 from .autograd import ground_energy  # noqa: F401
+from .disorder import disorder_average  # noqa: F401
+from . import petsc_io  # noqa: F401
 
 __version__ = "0.1.0"

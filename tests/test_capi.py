@@ -86,3 +86,18 @@ def test_random_fields_match_oracle(O):
     from lsquantumed_amd import fields_from_seed
     for seed in (1, 2, 77, 1000):
         assert np.array_equal(fields_from_seed(seed, 22, -2.0, 2.0), O.rd_hi(seed, 22, -2.0, 2.0))
+
+
+def test_petsc_format_helpers(O):
+    """The product's PETSc %g and row ownership equal the oracle's restatement
+    of the reference (petsc_g; Basis split, src/basis.cpp:78-80)."""
+    from lsquantumed_amd import petsc_io
+    vals = [0.0, 0.5, -1.5, 1.0, 2.0, -3.0, 9.83333333333, 1e-7, 123456789.0, 1.0 / 3.0, -0.25, 1e20]
+    for v in vals:
+        assert petsc_io.petsc_g(v) == O.petsc_g(v), v
+    for size in (1, 6, 7, 20, 495, 1001):
+        for P in (1, 2, 3, 5, 8):
+            for r in range(P):
+                assert petsc_io.ownership(size, P, r) == O.split(size, P, r), (size, P, r)
+    txt = petsc_io.vec_view([1.5], 1)
+    assert txt == "Vec Object: 1 MPI processes\n  type: seq\n1.5\n"

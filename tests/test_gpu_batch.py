@@ -246,3 +246,49 @@ def test_autograd_chain20_batch64(O, torch):
         np.testing.assert_allclose(hams[k].hi, h0[k], rtol=0, atol=0)
         ref = plain_e0(hams[k])
         assert abs(E[k] - ref) <= 1e-9 * abs(ref)
+
+
+def test_disorder_average_driver_and_resume(O, torch, tmp_path):
+    """The realisation loop of main_disorder.cpp with batched ground states:
+    every dsf_d<k>.dat has the ground energy of the oracle's dense spectrum,
+    excitation energies of the oracle, and multiplet-summed DSF weights of
+    the oracle's eigenvectors at every chain momentum; a deleted file is
+    recomputed on resume and the others are left alone."""
+    import os
+
+    import lsquantumed_amd as Q
+    from lsquantumed_amd.disorder import disorder_average
+    from test_oracle import _parse_dsf
+    N, nev = 8, 6
+    env = Q.Environment(N)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.chain1D(env), 1.0, 1.0, 0.2, 0.2)
+    h.build_diag(env)
+    h.build_off_diag(env)
+    rd = Q.RandomDisorder(N, True, -1.0, 1.0, 3)
+    done = disorder_average(env, h, rd, str(tmp_path), nev=nev)
+    assert sorted(done) == [0, 1, 2]
+    coords = np.array([[i, 0] for i in range(N)], float)
+    for k in range(3):
+        hi = rd.hi_init(k).copy()
+        m = O.Model(O.CHAIN, N, N // 2, 1.0, 1.0, 0.2, 0.2)
+        w, v = np.linalg.eigh(m.csr(hi, FLAGS_DIS).toarray())
+        gs, blocks = _parse_dsf(open(tmp_path / f"dsf_d{k}.dat").read())
+        assert abs(gs - w[0]) < 1e-11 and abs(done[k] - w[0]) < 1e-11
+        ex = np.array([e for e, _ in blocks[0]])
+        np.testing.assert_allclose(ex, w[:nev] - w[0], rtol=0, atol=1e-10)
+        amp = np.array([[blocks[i][n][1] for i in range(N)] for n in range(nev)])
+        ref_amp = O.dsf_amplitudes(N, N // 2, v[:, 0], v[:, :nev], "z")
+        groups = O.multiplets(w[:nev])
+        if abs(w[nev] - w[nev - 1]) <= 1e-8:
+            groups = groups[:-1]
+        for grp in groups:
+            for qx in range(N):
+                q = (2 * np.pi * qx / N, 0.0)
+                W = sum(O.dsf_weight(amp[n], q, coords) for n in grp)
+                Wr = sum(O.dsf_weight(ref_amp[n], q, coords) for n in grp)
+                assert abs(W - Wr) < 1e-9, (k, grp, qx, W, Wr)
+    t0 = os.path.getmtime(tmp_path / "dsf_d0.dat")
+    os.remove(tmp_path / "dsf_d1.dat")
+    again = disorder_average(env, h, rd, str(tmp_path), nev=nev)
+    assert sorted(again) == [1] and abs(again[1] - done[1]) < 1e-12
+    assert os.path.getmtime(tmp_path / "dsf_d0.dat") == t0
