@@ -12,6 +12,7 @@
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -420,6 +421,38 @@ void oq_csr_spmv(void* a, const double* x, double* y, int nthreads) {
     double acc = 0.0;
     for (int64_t q = A.rowptr[r]; q < A.rowptr[r + 1]; ++q) acc += A.vals[q] * x[A.cols[q]];
     y[r] = acc;
+  }
+}
+
+// The reference's data layout for timing: PETSc with complex scalars
+// (README.md:18) stores complex128 values and int64 columns (24 B per
+// nonzero) and multiplies complex vectors.  Built from the assembled CSR.
+struct CsrZ {
+  int64_t D = 0;
+  std::vector<int64_t> rowptr, cols;
+  std::vector<std::complex<double>> vals;
+};
+void* oq_csrz_from(void* a) {
+  Csr& A = *(Csr*)a;
+  CsrZ* Z = new CsrZ;
+  Z->D = A.D;
+  Z->rowptr = A.rowptr;
+  Z->cols = A.cols;
+  Z->vals.assign(A.vals.begin(), A.vals.end());
+  return Z;
+}
+void oq_csrz_free(void* z) { delete (CsrZ*)z; }
+// y = A x; x, y: D complex numbers as interleaved (re, im) doubles.
+void oq_csrz_spmv(void* z, const double* x, double* y, int nthreads) {
+  CsrZ& A = *(CsrZ*)z;
+  const std::complex<double>* xc = reinterpret_cast<const std::complex<double>*>(x);
+  std::complex<double>* yc = reinterpret_cast<std::complex<double>*>(y);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < A.D; ++r) {
+    std::complex<double> acc = 0.0;
+    for (int64_t q = A.rowptr[r]; q < A.rowptr[r + 1]; ++q) acc += A.vals[q] * xc[A.cols[q]];
+    yc[r] = acc;
   }
 }
 

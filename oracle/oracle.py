@@ -89,6 +89,9 @@ def lib():
             "oq_csr_copy": (None, [vp, vp, vp, vp]),
             "oq_csr_free": (None, [vp]),
             "oq_csr_spmv": (None, [vp, vp, vp, i32]),
+            "oq_csrz_from": (vp, [vp]),
+            "oq_csrz_free": (None, [vp]),
+            "oq_csrz_spmv": (None, [vp, vp, vp, i32]),
             "oq_hv_matfree": (None, [vp, vp, vp, vp, i64, i64, i32]),
             "oq_hv_rows_formula": (None, [vp, vp, vp, i64, i64, i32]),
             "oq_formula_x": (C.c_double, [i64]),
@@ -264,6 +267,61 @@ class Model:
             hp = _p(hi)
         lib().oq_hv_rows_formula(self.h, hp, _p(y), row0, nrows, nthreads)
         return y
+
+
+class ComplexCsr:
+    """The model's assembled matrix in the reference's layout (complex128
+    values, int64 columns), for the CPU baseline's SpMV."""
+
+    def __init__(self, model, hi=None, flags=Model.BUILD_DIAG | Model.BUILD_OFFDIAG):
+        hp = None
+        if hi is not None:
+            hi = np.ascontiguousarray(hi, np.float64)
+            hp = _p(hi)
+        a = lib().oq_csr_build(model.h, hp, flags)
+        self.D, self.nnz = lib().oq_csr_dim(a), lib().oq_csr_nnz(a)
+        self.z = lib().oq_csrz_from(a)
+        lib().oq_csr_free(a)
+
+    def __del__(self):
+        if getattr(self, "z", None):
+            lib().oq_csrz_free(self.z)
+            self.z = None
+
+    def matvec(self, x: np.ndarray, y: np.ndarray, nthreads: int = 0) -> None:
+        lib().oq_csrz_spmv(self.z, _p(x), _p(y), nthreads)
+
+
+def lanczos_ground_cpu(A: ComplexCsr, tol: float = 1e-9, maxit: int = 500, nthreads: int = 0, seed: int = 1):
+    """Three-term Lanczos on complex vectors with the CSR SpMV, stopped when
+    the lowest Ritz pair has |beta_m s_m| / |theta| < tol (checked every 5
+    steps).  Returns (E0, steps)."""
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(seed)
+    v = (rng.random(A.D) - 0.5).astype(np.complex128)
+    v /= np.linalg.norm(v)
+    vp = np.zeros_like(v)
+    w = np.empty_like(v)
+    al, be = [], []
+    beta = 0.0
+    theta = 0.0
+    for m in range(maxit):
+        A.matvec(v, w, nthreads)
+        a = np.vdot(v, w).real
+        w -= a * v
+        w -= beta * vp
+        beta = np.linalg.norm(w)
+        al.append(a)
+        be.append(beta)
+        if m % 5 == 4:
+            ev, s = sl.eigh_tridiagonal(np.array(al), np.array(be[:-1]), select="i", select_range=(0, 0))
+            theta = float(ev[0])
+            if abs(be[-1] * s[-1, 0]) < tol * abs(theta):
+                return theta, m + 1
+        vp, v = v, w / beta
+        w = np.empty_like(v)
+    return theta, maxit
 
 
 FORMULA_P = ((1000003, 7919), (999983, 104729))
