@@ -1041,32 +1041,62 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream) {
   b->steps.assign(np, 0);
   b->steps_vec.assign(np, 0);
   b->conv.assign(np, 0);
-  // state: 0 running, 1 retiring (its flag clears from the next enqueued replay), 2 done
-  std::vector<int> state(np, 0);
+  // Activity of each model in each enqueued replay is decided on the host:
+  //   done[k]      converged (or exhausted / out of steps): inactive from the
+  //                next enqueued replay on; the steps of the replay already in
+  //                flight are not used.
+  //   paused[k]    predicted to converge within the replay already in flight
+  //                (residual ratio of the last two replays): inactive from the
+  //                next enqueued replay; if the in-flight replay does not
+  //                confirm convergence it becomes active again.  A paused
+  //                model is untouched by every kernel, so pausing does not
+  //                change its Lanczos sequence, only saves the steps of a
+  //                replay it would not need.
+  std::vector<char> done(np, 0), paused(np, 0), cur(np, 1);
+  std::vector<double> prev_res(np, 0.0);
+  std::vector<std::vector<char>> act_at;  // activity vector of every enqueued replay
   const size_t HS = (size_t)2 * kSteps * np;  // doubles per history buffer
   int slot = 0;                               // ring slot of the next activity upload
+  bool dirty = false;                         // cur differs from the device flags
   std::vector<double> ritz;
-  auto enqueue = [&](int r) -> int {
+  auto enqueue = [&]() -> int {
+    const int r = (int)act_at.size();
+    if (dirty) {
+      int* act = b->h_active + (size_t)(slot % b->ring) * np;
+      ++slot;
+      for (int k = 0; k < np; ++k) act[k] = cur[k];
+      LSQED_HIP(hipMemcpyAsync(b->d_active, act, np * sizeof(int), hipMemcpyHostToDevice, b->gs));
+      dirty = false;
+    }
+    act_at.push_back(cur);
     const int rc2 = run_replay(b, 0);
     if (rc2) return rc2;
     LSQED_HIP(hipMemcpyAsync(b->h_hist + (r & 1) * HS, b->d_hist, HS * sizeof(double), hipMemcpyDeviceToHost, b->gs));
     LSQED_HIP(hipEventRecord(b->ev[r & 1], b->gs));
     return LSQED_OK;
   };
-  int enq = 0, done_r = 0, ndone = 0, nrun = np;
-  if ((rc = enqueue(enq++))) return rc;
+  auto any_active = [&]() {
+    for (int k = 0; k < np; ++k)
+      if (cur[k]) return true;
+    return false;
+  };
+  int done_r = 0, ndone = 0;
+  if ((rc = enqueue())) return rc;
   while (ndone < np) {
-    // keep one replay in flight beyond the one being processed while any
-    // problem is still running (retiring problems finish with the replay
-    // already in flight)
-    if (enq == done_r + 1 && nrun > 0)
-      if ((rc = enqueue(enq++))) return rc;
+    // keep one replay in flight beyond the one being processed (a model
+    // resumed after a failed prediction may need a replay when none is left)
+    if ((int)act_at.size() <= done_r + 1 && any_active())
+      if ((rc = enqueue())) return rc;
+    if ((int)act_at.size() <= done_r) {
+      (void)hipStreamSynchronize(b->gs);
+      return fail(LSQED_ERR_STATE, "batched Lanczos: no replay in flight");
+    }
     LSQED_HIP(hipEventSynchronize(b->ev[done_r & 1]));
     const double* ha = b->h_hist + (done_r & 1) * HS;
     const double* hb = ha + (size_t)kSteps * np;
-    bool changed = false;
+    const std::vector<char>& was = act_at[done_r];
     for (int k = 0; k < np; ++k) {
-      if (state[k] == 2) continue;
+      if (done[k] || !was[k]) continue;
       auto& al = b->al[k];
       auto& be = b->be[k];
       bool broke = false;
@@ -1080,42 +1110,32 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream) {
       }
       const int m = (int)al.size();
       const double theta = tri_lowest(al.data(), be.data(), m, &ritz);
-      const double res = std::fabs(be.back() * ritz[m - 1]);
+      const double res = std::fabs(be.back() * ritz[m - 1]) / std::max(std::fabs(theta), 1e-300);
       b->e0[k] = theta;
-      b->resid[k] = res / std::max(std::fabs(theta), 1e-300);
+      b->resid[k] = res;
       b->steps[k] = m;
-      const bool conv = broke || res < b->opts.tol * std::fabs(theta);
-      if (state[k] == 1 || broke) {
-        if (!b->steps_vec[k]) b->steps_vec[k] = m;
-        if (conv) b->conv[k] = 1;
-        if (state[k] == 0) {
-          changed = true;
-          --nrun;
-        }
-        state[k] = 2;
-        ++ndone;
-      } else if (conv || m + kSteps > b->opts.maxit) {
-        // the Ritz vector is taken at this step count; the steps of the
-        // replay already in flight only refine the eigenvalue
+      const bool conv = broke || res < b->opts.tol;
+      if (conv || m + kSteps > b->opts.maxit) {
         b->steps_vec[k] = m;
-        if (conv) b->conv[k] = 1;
-        state[k] = 1;
-        changed = true;
-        --nrun;
+        b->conv[k] = conv ? 1 : 0;
+        done[k] = 1;
+        ++ndone;
+        if (cur[k]) dirty = true;
+        cur[k] = 0;
+        paused[k] = 0;
+      } else if (paused[k]) {
+        // the prediction failed: resume
+        paused[k] = 0;
+        cur[k] = 1;
+        dirty = true;
+      } else if (prev_res[k] > 0.0 && res < prev_res[k] && res * (res / prev_res[k]) < 0.5 * b->opts.tol) {
+        paused[k] = 1;
+        cur[k] = 0;
+        dirty = true;
       }
+      prev_res[k] = res;
     }
     ++done_r;
-    if (ndone < np && enq == done_r) {
-      // retiring problems always have their last replay in flight
-      (void)hipStreamSynchronize(b->gs);
-      return fail(LSQED_ERR_STATE, "batched Lanczos: no replay in flight");
-    }
-    if (changed && ndone < np) {
-      int* act = b->h_active + (size_t)(slot % b->ring) * np;
-      ++slot;
-      for (int k = 0; k < np; ++k) act[k] = state[k] == 0;
-      LSQED_HIP(hipMemcpyAsync(b->d_active, act, np * sizeof(int), hipMemcpyHostToDevice, b->gs));
-    }
   }
   LSQED_HIP(hipStreamSynchronize(b->gs));
   for (int k = 0; k < np; ++k) e0[k] = b->e0[k];
