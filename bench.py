@@ -249,6 +249,87 @@ def cpu_disorder_baseline():
             "assembly_s": {str(N): assembly[N] for N in sizes}}
 
 
+def chain16(reps: int = 5):
+    """Periodic chain N=16, J1=D1=1, Sz=0 (D = 12,870): Krylov-Schur nev=1,
+    tol 1e-9, E0 and <S^z_0 S^z_r> for all r; wall time of SolverInit +
+    solve + solution + the correlators, after one untimed run."""
+    import numpy as np
+    import torch
+
+    import lsquantumed_amd as Q
+
+    env = Q.Environment(16)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.chain1D(env), 1.0, 1.0, 0.0, 0.0)
+    h.build_diag(env)
+    h.build_off_diag(env)
+    psi = torch.empty(h.hamilt_size(), dtype=torch.float64, device="cuda")
+
+    def run():
+        eps = Q.Solver.SolverInit(h, 1, tol=1e-9)
+        Q.Solver.solve(env, eps)
+        lam, err = Q.Solver.solution(eps, 0, psi)
+        Q.Solver.SolverClean(eps)
+        return lam, err, Q.correlations(h, psi)["SzSz"][0]
+
+    run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        lam, err, zz = run()
+    dt = (time.perf_counter() - t) / reps
+    return {"workload": "periodic chain N=16, J1=D1=1, Sz=0, D=12870, Krylov-Schur nev=1 tol 1e-9",
+            "E0": lam, "relative_error": err, "SzSz_0r": [float(v) for v in np.asarray(zz)],
+            "wall_s": round(dt, 5)}
+
+
+def hv_4x6_with_cpu():
+    """H.v on the periodic 4x6 J1-J2 sector (D = 2,704,156): GPU (HIP events,
+    the automatically chosen kernel) and CPU (the oracle's complex128 CSR in
+    PETSc's layout, OpenMP SpMV over the host cores); GB/s by the algorithmic
+    bytes B_Hv, and for the CPU also with the CSR bytes (24 B per stored
+    nonzero + 16 B per complex gather and per row)."""
+    import numpy as np
+    import torch
+
+    import lsquantumed_amd as Q
+    from oracle import oracle as O
+
+    env = Q.Environment(24)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.square2D(env, 4, 6), 1.0, 1.0, 0.5, 0.5)
+    h.build_diag(env)
+    h.build_off_diag(env)
+    B, D, nnz = algorithmic_bytes(24, 12, 96)
+    x = torch.rand(D, dtype=torch.float64, device="cuda") - 0.5
+    y = torch.empty_like(x)
+    h.matvec(x, y)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(20):
+        h.matvec(x, y)
+    ev1.record()
+    torch.cuda.synchronize()
+    gpu_ms = ev0.elapsed_time(ev1) / 20
+    threads = O.lib().oq_max_threads()
+    t = time.perf_counter()
+    A = O.ComplexCsr(O.Model(O.SQUARE, 24, 12, 1.0, 1.0, 0.5, 0.5, lx=4, ly=6))
+    build_s = time.perf_counter() - t
+    xc = (np.random.default_rng(0).random(D) - 0.5).astype(np.complex128)
+    yc = np.empty_like(xc)
+    A.matvec(xc, yc, threads)
+    t = time.perf_counter()
+    for _ in range(3):
+        A.matvec(xc, yc, threads)
+    cpu_s = (time.perf_counter() - t) / 3
+    csr_bytes = 24 * A.nnz + 16 * A.nnz + 16 * D
+    return {"workload": "H.v, periodic square 4x6 (N=24), J1=D1=1, J2=D2=0.5, Sz=0, D=2704156",
+            "algorithmic_bytes": B, "gpu_ms": round(gpu_ms, 4), "gpu_gbs": round(B / gpu_ms / 1e6, 1),
+            "gpu_kernel": {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()],
+            "cpu_ms": round(cpu_s * 1e3, 2), "cpu_gbs": round(B / cpu_s / 1e9, 2),
+            "cpu_csr_gbs": round(csr_bytes / cpu_s / 1e9, 2), "cpu_threads": threads,
+            "cpu_csr_nnz": A.nnz, "cpu_csr_build_s": round(build_s, 2),
+            "cpu_kind": "oracle complex128 CSR SpMV (PETSc's layout), OpenMP"}
+
+
 def krylov_schur_4x6():
     """Krylov-Schur nev=10, ncv=32, tol 1e-9 on the periodic 4x6 square
     lattice, J1=D1=1, J2=D2=0.5, Sz=0 (D = 2,704,156): wall time of
@@ -486,6 +567,8 @@ def main():
     if batch is not None and not args.no_cpu:
         batch["cpu_baseline"] = cpu_disorder_baseline()
     ks = None if args.no_batch else krylov_schur_4x6()
+    c1 = None if args.no_batch else chain16()
+    hv46 = None if (args.no_batch or args.no_cpu) else hv_4x6_with_cpu()
     cpu = None if args.no_cpu else cpu_baseline((N, N // 2, J1, D1, J2, D2, lx, ly, B))
     out = {
         "metric": METRIC,
@@ -515,6 +598,8 @@ def main():
         "cpu_baseline": cpu,
         "batched_disorder": batch,
         "krylov_schur_4x6": ks,
+        "hv_4x6": hv46,
+        "chain16": c1,
         "alpha_last": alphas[-1],
     }
     print(json.dumps(out))
