@@ -292,3 +292,29 @@ def test_disorder_average_driver_and_resume(O, torch, tmp_path):
     again = disorder_average(env, h, rd, str(tmp_path), nev=nev)
     assert sorted(again) == [1] and abs(again[1] - done[1]) < 1e-12
     assert os.path.getmtime(tmp_path / "dsf_d0.dat") == t0
+
+
+def test_batch_mixed_lattices_and_sectors(O, torch):
+    """One batch holding a 3x4 square lattice (Sz = 0 and Sz = 1), a 6x2
+    honeycomb with J2 and a 10-site chain in Sz = -1, with and without
+    fields: each model its own specialised-kernel group; every E0 equals the
+    oracle's dense spectrum to 1e-11, for both H.v kernels."""
+    import lsquantumed_amd as Q
+    from lsquantumed_amd.batch import BatchSolver
+    from test_gpu_hamiltonian import make
+    specs = [("square", 12, 0, 1.0, 1.0, 0.5, 0.5, 3, 4, None), ("square", 12, 1, 1.0, 0.7, 0.0, 0.0, 3, 4, 0.3),
+             ("honeycomb", 12, 0, 1.0, 1.0, 0.4, 0.4, 6, 2, 0.8), ("chain", 10, -1, 1.0, 1.0, 0.2, 0.2, 0, 0, 1.5)]
+    lt = {"chain": O.CHAIN, "square": O.SQUARE, "honeycomb": O.HONEYCOMB}
+    hams, refs = [], []
+    for k, (lat, N, mag, J1, D1, J2, D2, lx, ly, W) in enumerate(specs):
+        hi = None if W is None else Q.fields_from_seed(40 + k, N, -W, W)
+        builds = ("build_diag", "build_off_diag") + (("build_disorder",) if hi is not None else ())
+        hams.append(make(lat, N, mag, J1, D1, J2, D2, lx, ly, builds=builds, hi=hi))
+        m = O.Model(lt[lat], N, N // 2 + mag, J1, D1, J2, D2, lx=lx, ly=ly)
+        flags = 3 | (4 if hi is not None else 0)
+        refs.append(np.linalg.eigvalsh(m.csr(hi, flags).toarray())[0])
+    for kern in (1, 2):
+        say("mixed batch, kernel", kern)
+        e0 = BatchSolver(hams, tol=1e-12, hv_kernel=kern).solve()
+        for k in range(len(hams)):
+            assert abs(e0[k] - refs[k]) <= 1e-11 * abs(refs[k]), (kern, k, e0[k], refs[k])
