@@ -591,7 +591,7 @@ int setup_groups(lsqed_batch* b) {
 }
 
 int grid_for(const lsqed_batch* b, int64_t work_items) {
-  int64_t g = (int64_t)b->cus * b->shape.bpc;
+  int64_t g = (int64_t)b->cus * (int64_t)b->shape.bpc;
   g = std::min<int64_t>(g, std::max<int64_t>(work_items, 1));
   return (int)std::max<int64_t>(8, ((g + 7) / 8) * 8);
 }
@@ -897,7 +897,7 @@ int autotune(lsqed_batch* b) {
     }
   }
   if (try_spec)
-    for (int bpc : {2, 4, 8})
+    for (int bpc : {2, 4, 8, 32, 1 << 20})  // 1 << 20: one work item per block
       if ((rc = trial(Shape{1, rows_best, 1, bpc}))) return rc;
   b->shape = best;
   std::lock_guard<std::mutex> lk(g_tune_mu);
