@@ -330,6 +330,45 @@ def hv_4x6_with_cpu():
             "cpu_kind": "oracle complex128 CSR SpMV (PETSc's layout), OpenMP"}
 
 
+def autograd_chain20(B: int = 64, N: int = 20):
+    """Differentiable ground energy: 64 periodic chains of 20 spins, J1=1,
+    J2=0.3 (D_k = J_k), h_i ~ U[-1, 1] from mt19937_64(1 + k); wall time of
+    forward (batched solve with ground states) + backward (batched
+    observables kernel), after one untimed pass; central differences
+    (step 1e-5) of sum(E0) in J1 and J2 against the gradient."""
+    import numpy as np
+    import torch
+
+    from lsquantumed_amd import ground_energy
+    from lsquantumed_amd.tools import fields_from_seed
+
+    h0 = np.stack([fields_from_seed(1 + k, N, -1.0, 1.0) for k in range(B)])
+
+    def fwd_bwd():
+        h = torch.tensor(h0, dtype=torch.float64, requires_grad=True)
+        J1 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        J2 = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+        E = ground_energy(h, J1, J2)
+        E.sum().backward()
+        return E, h.grad, J1.grad.item(), J2.grad.item()
+
+    fwd_bwd()
+    t = time.perf_counter()
+    E, gh, gj1, gj2 = fwd_bwd()
+    wall = time.perf_counter() - t
+    d = 1e-5
+    hh = torch.tensor(h0)
+    with torch.no_grad():
+        fj1 = (ground_energy(hh, 1.0 + d, 0.3).sum() - ground_energy(hh, 1.0 - d, 0.3).sum()).item() / (2 * d)
+        fj2 = (ground_energy(hh, 1.0, 0.3 + d).sum() - ground_energy(hh, 1.0, 0.3 - d).sum()).item() / (2 * d)
+    return {"workload": f"E0(h, J1, J2) and its gradient, {B} periodic chains of N={N}, J1=1, J2=0.3, "
+                        "h_i~U[-1,1] from mt19937_64(1+k)",
+            "forward_backward_s": round(wall, 4), "E0_mean": float(E.mean()),
+            "dE_dJ1": gj1, "dE_dJ1_fd": fj1, "dE_dJ1_rel_err": abs(gj1 - fj1) / abs(gj1),
+            "dE_dJ2": gj2, "dE_dJ2_fd": fj2, "dE_dJ2_rel_err": abs(gj2 - fj2) / abs(gj2),
+            "sum_dE_dh": float(gh.sum())}
+
+
 def krylov_schur_4x6():
     """Krylov-Schur nev=10, ncv=32, tol 1e-9 on the periodic 4x6 square
     lattice, J1=D1=1, J2=D2=0.5, Sz=0 (D = 2,704,156): wall time of
@@ -568,6 +607,7 @@ def main():
         batch["cpu_baseline"] = cpu_disorder_baseline()
     ks = None if args.no_batch else krylov_schur_4x6()
     c1 = None if args.no_batch else chain16()
+    ag = None if args.no_batch else autograd_chain20()
     hv46 = None if (args.no_batch or args.no_cpu) else hv_4x6_with_cpu()
     cpu = None if args.no_cpu else cpu_baseline((N, N // 2, J1, D1, J2, D2, lx, ly, B))
     out = {
@@ -600,6 +640,7 @@ def main():
         "krylov_schur_4x6": ks,
         "hv_4x6": hv46,
         "chain16": c1,
+        "autograd_chain20": ag,
         "alpha_last": alphas[-1],
     }
     print(json.dumps(out))
