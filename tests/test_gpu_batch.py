@@ -318,3 +318,43 @@ def test_batch_mixed_lattices_and_sectors(O, torch):
         e0 = BatchSolver(hams, tol=1e-12, hv_kernel=kern).solve()
         for k in range(len(hams)):
             assert abs(e0[k] - refs[k]) <= 1e-11 * abs(refs[k]), (kern, k, e0[k], refs[k])
+
+
+def test_disorder_average_sqp_chunked(O, torch, tmp_path):
+    """The driver with Sqp (S^+_i from Sz = 0 into Sz = 1, the target sector
+    carrying the same fields) and batches smaller than the realisation count:
+    energies and multiplet-summed weights of every file against the oracle."""
+    import lsquantumed_amd as Q
+    from lsquantumed_amd.disorder import disorder_average
+    from test_oracle import _parse_dsf
+    N, nev = 6, 5
+    env = Q.Environment(N)
+    lat = Q.chain1D(env)
+    h0 = Q.Hamiltonian(env, Q.Basis(env, 0), lat, 1.0, 1.0, 0.0, 0.0)
+    h1 = Q.Hamiltonian(env, Q.Basis(env, 1), lat, 1.0, 1.0, 0.0, 0.0)
+    for h in (h0, h1):
+        h.build_diag(env)
+        h.build_off_diag(env)
+    rd = Q.RandomDisorder(N, True, -0.5, 0.5, 3)
+    done = disorder_average(env, h0, rd, str(tmp_path), nev=nev, op="Sqp", h1=h1, batch_size=2)
+    assert sorted(done) == [0, 1, 2]
+    coords = np.array([[i, 0] for i in range(N)], float)
+    for k in range(3):
+        hi = rd.hi_init(k).copy()
+        w0, v0 = np.linalg.eigh(O.Model(O.CHAIN, N, 3, 1.0, 1.0, 0.0, 0.0).csr(hi, 7).toarray())
+        w1, v1 = np.linalg.eigh(O.Model(O.CHAIN, N, 4, 1.0, 1.0, 0.0, 0.0).csr(hi, 7).toarray())
+        gs, blocks = _parse_dsf(open(tmp_path / f"dsf_d{k}.dat").read())
+        assert abs(gs - w0[0]) < 1e-11
+        ex = np.array([e for e, _ in blocks[0]])
+        np.testing.assert_allclose(ex, w1[:nev] - w0[0], rtol=0, atol=1e-10)
+        amp = np.array([[blocks[i][n][1] for i in range(N)] for n in range(nev)])
+        ref = O.dsf_amplitudes(N, 3, v0[:, 0], v1[:, :nev], "p")
+        groups = O.multiplets(w1[:nev])
+        if abs(w1[nev] - w1[nev - 1]) <= 1e-8:
+            groups = groups[:-1]
+        for grp in groups:
+            for qx in range(N):
+                q = (2 * np.pi * qx / N, 0.0)
+                W = sum(O.dsf_weight(amp[n], q, coords) for n in grp)
+                Wr = sum(O.dsf_weight(ref[n], q, coords) for n in grp)
+                assert abs(W - Wr) < 1e-9, (k, grp, qx)
