@@ -243,3 +243,38 @@ def test_square4x6_j1j2_nev10(O, torch):
         d_dsf = max(d_dsf, abs(W - grp["weight"]))
     say("max |d SiSj|", d_sisj, "|d m2|", abs(m2 - fx["m2_af"]), "max |d W(pi,pi)|", d_dsf)
     assert d_sisj <= 1e-8 and abs(m2 - fx["m2_af"]) <= 1e-8 and d_dsf <= 1e-8
+
+
+def test_square4x8_ground_state_sum_rules(O, torch):
+    """4x8 J1-J2 (D = 601,080,390) ground state from the batched solver (one
+    model, vectors kept): E0 matches the three-vector Lanczos to 1e-10; the
+    all-pairs correlators satisfy E0 = J1 sum_nn <S_i.S_j> + J2 sum_nnn
+    <S_i.S_j> (1e-9 relative) and the singlet sum rule sum_j <S_i.S_j> = 0
+    (1e-8); the batched observables kernel gives the same bond sums (1e-10)."""
+    import time
+
+    import lsquantumed_amd as Q
+    from lsquantumed_amd import correlations, lanczos_ground
+    from lsquantumed_amd.batch import OBS_FLIP_NN, OBS_FLIP_NNN, OBS_ZZ_NN, OBS_ZZ_NNN, BatchSolver
+    env = Q.Environment(32)
+    lat = Q.square2D(env, 4, 8)
+    h = make("square", 32, 0, 1.0, 1.0, 0.5, 0.5, 4, 8)
+    t = time.perf_counter()
+    b = BatchSolver([h], tol=1e-10, keep_vectors=True)
+    e0 = b.solve()[0]
+    obs = b.observables()[0]
+    psi = b.ground_state(0)
+    say("4x8 batched ground state", e0, b.info(0), round(time.perf_counter() - t, 2), "s")
+    b.destroy()
+    e3, steps = lanczos_ground(h, maxit=400, tol=1e-13)
+    assert abs(e0 - e3) <= 1e-10 * abs(e3), (e0, e3)
+    t = time.perf_counter()
+    c = correlations(h, psi)
+    say("4x8 correlators", round(time.perf_counter() - t, 2), "s")
+    S = c["SiSj"]
+    nn = sum(S[i, j] for i, j in zip(*lat.nn_bonds))
+    nnn = sum(S[i, j] for i, j in zip(*lat.nnn_bonds))
+    assert abs(1.0 * nn + 0.5 * nnn - e0) <= 1e-9 * abs(e0), (nn, nnn, e0)
+    assert np.max(np.abs(S.sum(axis=1))) <= 1e-8
+    assert abs(obs[OBS_ZZ_NN] + obs[OBS_FLIP_NN] - nn) <= 1e-10 * abs(nn)
+    assert abs(obs[OBS_ZZ_NNN] + obs[OBS_FLIP_NNN] - nnn) <= 1e-10 * max(abs(nnn), 1.0)
