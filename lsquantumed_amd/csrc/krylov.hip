@@ -290,41 +290,57 @@ static int finish_reduce(lsqed_ham* h, const double* part, int n, int ncol, doub
 // ------------------------------------------------- small dense eigensolver --
 // Cyclic Jacobi for a symmetric n x n matrix A (row-major, destroyed):
 // eigenvalues ascending in w, eigenvectors as columns of Z (row-major n x n).
+// Each rotation (p, q) sets a_pq to zero and moves the diagonal by t a_pq,
+// t = tan of the rotation angle (Golub & Van Loan, Matrix Computations,
+// sec. 8.5); only the entries off rows/columns p, q go through the rotation
+// itself.  Eigenvalues come out accurate to a few ulp of ||A||, which the
+// Ritz values of a converged Krylov-Schur solve inherit.
 static void sym_eig(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z) {
   Z.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
   auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
   for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        tot += a(i, j) * a(i, j);
-        if (i != j) off += a(i, j) * a(i, j);
-      }
-    if (off <= 1e-32 * tot || off == 0.0) break;
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i) {
+      diag += a(i, i) * a(i, i);
+      for (int j = i + 1; j < n; ++j) off += a(i, j) * a(i, j);
+    }
+    if (off == 0.0 || off <= 1e-36 * diag) break;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
         const double apq = a(p, q);
-        if (std::fabs(apq) < 1e-300) continue;
+        if (apq == 0.0) continue;
+        if (std::fabs(apq) <= 1e-18 * (std::fabs(a(p, p)) + std::fabs(a(q, q)))) {
+          a(p, q) = a(q, p) = 0.0;
+          continue;
+        }
         const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
         const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
         for (int k = 0; k < n; ++k) {
+          if (k == p || k == q) continue;
           const double akp = a(k, p), akq = a(k, q);
-          a(k, p) = c * akp - s * akq;
-          a(k, q) = s * akp + c * akq;
+          a(k, p) = a(p, k) = c * akp - s * akq;
+          a(k, q) = a(q, k) = s * akp + c * akq;
         }
-        for (int k = 0; k < n; ++k) {
-          const double apk = a(p, k), aqk = a(q, k);
-          a(p, k) = c * apk - s * aqk;
-          a(q, k) = s * apk + c * aqk;
-        }
+        a(p, p) -= t * apq;
+        a(q, q) += t * apq;
+        a(p, q) = a(q, p) = 0.0;
         for (int k = 0; k < n; ++k) {
           const double zkp = Z[(size_t)k * n + p], zkq = Z[(size_t)k * n + q];
           Z[(size_t)k * n + p] = c * zkp - s * zkq;
           Z[(size_t)k * n + q] = s * zkp + c * zkq;
         }
       }
+  }
+  // unit columns: the accumulated rotations drift from orthonormal by a few
+  // ulp per sweep, and a Ritz vector's norm error becomes a relative error
+  // of its Ritz value
+  for (int c = 0; c < n; ++c) {
+    double ss = 0.0;
+    for (int r = 0; r < n; ++r) ss += Z[(size_t)r * n + c] * Z[(size_t)r * n + c];
+    const double inv = 1.0 / std::sqrt(ss);
+    for (int r = 0; r < n; ++r) Z[(size_t)r * n + c] *= inv;
   }
   std::vector<int> idx(n);
   for (int i = 0; i < n; ++i) idx[i] = i;
