@@ -24,10 +24,11 @@ class Environment:
     and SLEPc initialisation have no counterpart on one GPU."""
 
     def __init__(self, nspins: int):
-        if nspins <= 0:
-            raise ValueError("The number of spins has to be a positive number.")
+        # odd counts first, then non-positive ones, with the reference's messages
         if nspins % 2 != 0:
-            raise ValueError("The number of spins has to be an even number.")
+            raise ValueError("The total number of spins has to be EVEN.")
+        if nspins <= 0:
+            raise ValueError("The total number of spins has to be POSITIVE.")
         self.nspins = int(nspins)
         self.mpi_rank = 0
         self.mpi_size = 1
@@ -93,14 +94,30 @@ class chain1D(_Lattice):
         super().__init__(env, 0, 0)
 
 
+_PRODUCT_MSG = ("The product between the number of spins in x direction and the numer of spins in the y direction "
+                "has to be equal to the total number of spins.")
+
+
 class square2D(_Lattice):
     """square2D(env, lx, ly) (src/lattice.cpp:59-136)."""
     type_id = _lib.SQUARE2D
+
+    def __init__(self, env, lx: int, ly: int):
+        if lx * ly != _nspins(env):
+            raise ValueError(_PRODUCT_MSG)
+        super().__init__(env, lx, ly)
 
 
 class honeycomb2D(_Lattice):
     """honeycomb2D(env, lx, ly) (src/lattice.cpp:142-264)."""
     type_id = _lib.HONEYCOMB2D
+
+    def __init__(self, env, lx: int, ly: int):
+        if lx * ly != _nspins(env):
+            raise ValueError(_PRODUCT_MSG)
+        if lx % 2 != 0 or ly % 2 != 0:
+            raise ValueError("The number of spins in each direction has to be even.")
+        super().__init__(env, lx, ly)
 
 
 class AF:

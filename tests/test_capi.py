@@ -54,16 +54,26 @@ def test_bonds_match_oracle(O, lat, N, lx, ly):
         assert list(got[0]) == list(bi) and list(got[1]) == list(bj)
 
 
-def test_lattice_errors():
-    from lsquantumed_amd import LsqedError, model
-    with pytest.raises(LsqedError):
+def test_lattice_errors(L):
+    """The reference's exceptions and messages (src/environment.cpp:31-35,
+    src/basis.cpp:66-67, src/lattice.cpp:67-68, :150-154); the C-ABI itself
+    rejects the same inputs with LSQED_ERR_ARG."""
+    from lsquantumed_amd import model
+    with pytest.raises(ValueError, match="^The product between the number of spins"):
         model.square2D(12, 3, 5)
-    with pytest.raises(LsqedError):
+    with pytest.raises(ValueError, match="^The number of spins in each direction has to be even.$"):
         model.honeycomb2D(12, 3, 4)
-    with pytest.raises(ValueError):
+    with pytest.raises(ValueError, match="^The total number of spins has to be EVEN.$"):
         model.Environment(7)
-    with pytest.raises(ValueError):
+    with pytest.raises(ValueError, match="^The total number of spins has to be EVEN.$"):
+        model.Environment(-3)
+    with pytest.raises(ValueError, match="^The total number of spins has to be POSITIVE.$"):
+        model.Environment(0)
+    with pytest.raises(ValueError, match="^The required constant magnetization has to be <= nspins/2.$"):
         model.Basis(8, 5)
+    n = C.c_int()
+    assert L.lsqed_lattice_bonds(LT["square"], 12, 3, 5, 0, None, None, C.byref(n)) == -1
+    assert L.lsqed_lattice_bonds(LT["honeycomb"], 12, 3, 4, 0, None, None, C.byref(n)) == -1
 
 
 def test_sector_dim(L):
