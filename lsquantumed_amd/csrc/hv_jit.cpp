@@ -528,8 +528,8 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
   a.RL1 = t.RL1;
   a.binom = t.binom;
   a.binom64 = t.binom64;
-  a.tasks = t.tasks;
-  a.ntasks = t.ntasks;
+  a.tasks = t.tasks_ord;
+  a.ntasks = t.ntasks_ord;
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
   void* args[] = {&a};
   LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, args,

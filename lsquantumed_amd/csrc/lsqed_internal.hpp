@@ -130,6 +130,8 @@ struct SplitTables {
   const uint32_t *binom, *tasks;
   const uint64_t* binom64;
   int ntasks;
+  const uint32_t* tasks_ord;  // the items of one H.v in cache-reuse order (split_grid blocks)
+  int ntasks_ord;
   const int* offU;
 };
 bool split_supported(const lsqed_ham* h);
