@@ -6,8 +6,8 @@
  * SLEPc: `Hamiltonian<L>::hamilt` (include/hamiltonian.hpp:187), created as
  * MATMPIAIJ in `MatInit` (include/hamiltonian.tmpl_func.hpp:8-56) and passed
  * to `Solver::SolverInit` (src/solver.cpp:7-24).  A PETSc MATSHELL whose
- * MATOP_MULT calls lsqed_hv() is the drop-in on the reference side; the
- * binding is shown in INTEGRATION.md.
+ * MATOP_MULT calls lsqed_hv() and MATOP_GET_DIAGONAL calls lsqed_diag() is
+ * the drop-in on the reference side.
  *
  * Conventions
  *  - Every call returns an int status: LSQED_OK (0) or a negative LSQED_ERR_*
@@ -107,8 +107,9 @@ int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
  * hipRTC is available, else the row split when N <= 32, else the generic
  * chunk-table kernel), 1 = generic chunk-table kernel, 2 = row-split kernel
  * (N <= 32), 3 = row split specialised to the lattice (compiled once per
- * bond structure with hipRTC; 64-bit row bases above N = 32).  No counterpart in the reference (whose operator is an assembled
- * matrix); used by the tests to check every kernel and by the tuner.
+ * bond structure with hipRTC; 64-bit row bases above N = 32).  No
+ * counterpart in the reference (whose operator is an assembled matrix); used
+ * by the tests to check every kernel and by the tuner.
  * Setting LSQED_JIT=0 in the environment keeps automatic mode off hipRTC. */
 int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant);
 /* The kernel lsqed_hv will use (1, 2 or 3), resolving automatic mode. */
