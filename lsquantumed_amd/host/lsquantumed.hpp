@@ -811,6 +811,80 @@ PetscErrorCode DynStructFactor(Environment& env, DSF_data<L>& data, PetscScalar 
 
 }  // namespace Phys
 
+/* -------------------------------------------------------- batched solver */
+/* The realisation loop of examples/main_disorder.cpp:82-120 (per
+ * realisation: build_disorder, SolverInit with nev = 1, solve, solution) as
+ * one batched three-term Lanczos over many models (lsqed_batch_*).  The
+ * models are referenced, not copied: build_disorder on them between solves
+ * takes effect at the next solve.  No counterpart in the reference, which
+ * solves the realisations one at a time. */
+template <typename L>
+class BatchSolver {
+ public:
+  struct Options {
+    PetscInt maxit = 300;
+    PetscReal tol = 1e-9;
+    uint64_t seed = 1;
+    PetscBool keep_vectors = PETSC_FALSE;  // a second pass builds the ground states
+  };
+
+  explicit BatchSolver(const std::vector<Hamiltonian<L>*>& models) : BatchSolver(models, Options{}) {}
+  BatchSolver(const std::vector<Hamiltonian<L>*>& models, const Options& o) {
+    if (models.empty()) throw std::invalid_argument("the batch needs at least one model");
+    for (Hamiltonian<L>* m : models) {
+      if (!m || !m->hamilt) throw std::invalid_argument("model without an operator");
+      hams.push_back(m->hamilt);
+    }
+    lsqed_batch_opts bo{};
+    bo.maxit = int(o.maxit);
+    bo.tol = o.tol;
+    bo.seed = o.seed;
+    bo.keep_vectors = o.keep_vectors ? 1 : 0;
+    detail::check(lsqed_batch_create(int(hams.size()), hams.data(), &bo, &batch));
+  }
+  ~BatchSolver() {
+    if (batch) lsqed_batch_destroy(batch);
+  }
+  BatchSolver(const BatchSolver&) = delete;
+  BatchSolver& operator=(const BatchSolver&) = delete;
+
+  PetscInt size() const { return PetscInt(hams.size()); }
+
+  /* Lowest eigenvalue of every model, in the order given. */
+  PetscErrorCode solve(std::vector<PetscScalar>& e0) {
+    e0.assign(hams.size(), 0.0);
+    return lsqed_batch_solve(batch, e0.data(), nullptr);
+  }
+  /* Lanczos steps, relative residual |beta_m s_m| / |theta| and convergence of model k. */
+  PetscErrorCode info(PetscInt k, PetscInt& steps, PetscReal& resid, PetscBool& converged) const {
+    int s = 0, c = 0;
+    double r = 0.0;
+    const int rc = lsqed_batch_info(batch, int(k), &s, &r, &c);
+    steps = s;
+    resid = r;
+    converged = c != 0;
+    return rc;
+  }
+  /* Unit-norm ground state of model k (needs keep_vectors) into a vector of
+   * its sector (MatCreateVecs on that model). */
+  PetscErrorCode ground_state(PetscInt k, Vec& state) {
+    if (!state || k < 0 || k >= size() || state->n != lsqed_ham_dim(hams[size_t(k)])) return LSQED_ERR_ARG;
+    ++state->state;
+    return lsqed_batch_ground_state(batch, int(k), state->array, nullptr);
+  }
+  /* dE0/dparameter of every model from its ground state (needs
+   * keep_vectors): out[k * LSQED_BATCH_OBS_STRIDE + j], layout of
+   * lsqed_batch_observables. */
+  PetscErrorCode observables(std::vector<double>& out) {
+    out.assign(hams.size() * LSQED_BATCH_OBS_STRIDE, 0.0);
+    return lsqed_batch_observables(batch, out.data(), nullptr);
+  }
+
+ private:
+  std::vector<lsqed_ham*> hams;
+  lsqed_batch* batch = nullptr;
+};
+
 /* ------------------------------------------------------------------ Tools */
 namespace Tools {
 

@@ -538,6 +538,87 @@ static void dsf_case(bool raise) {
   EXPECT_EQ(0, VecDestroy(&xi));
 }
 
+/* The disorder loop of examples/main_disorder.cpp:82-120 through the
+ * batched solver: eight realisations of a 12-site J1-J2 chain in random
+ * fields against the per-realisation Krylov-Schur, ground states against
+ * their Rayleigh quotient and <S^z_i>, and the Hellmann-Feynman derivatives
+ * against Euler's identity E0 = sum_p p dE0/dp (H is linear in J1, D1, J2,
+ * D2 and the fields). */
+TEST(BatchSolver, DisorderChains) {
+  Environment env12{_argc, _argv, 12, help};
+  Basis b{env12, 0};
+  chain1D lat{env12};
+  const double J1 = 1.0, D1 = 1.0, J2 = 0.2, D2 = 0.2;
+  Tools::RandomDisorder rd{12, PETSC_TRUE, -2.0, 2.0, 8};
+  std::vector<std::unique_ptr<Hamiltonian<chain1D>>> owned;
+  std::vector<Hamiltonian<chain1D>*> models;
+  std::vector<std::vector<double>> fields;
+  for (PetscInt k = 0; k < rd.rep; ++k) {
+    owned.emplace_back(new Hamiltonian<chain1D>{env12, b, lat, J1, D1, J2, D2});
+    ASSERT_EQ(0, owned.back()->build_diag(env12));
+    ASSERT_EQ(0, owned.back()->build_off_diag(env12));
+    rd.hi_init(k);
+    fields.push_back(rd.hi);
+    ASSERT_EQ(0, owned.back()->build_disorder(env12, rd.hi));
+    models.push_back(owned.back().get());
+  }
+  BatchSolver<chain1D>::Options o;
+  o.tol = 1e-10;
+  o.keep_vectors = PETSC_TRUE;
+  BatchSolver<chain1D> batch{models, o};
+  std::vector<PetscScalar> e0;
+  ASSERT_EQ(0, batch.solve(e0));
+  std::vector<double> obs;
+  ASSERT_EQ(0, batch.observables(obs));
+  Vec psi, hpsi;
+  ASSERT_EQ(0, MatCreateVecs(models[0]->hamilt, &psi, &hpsi));
+  const size_t D = size_t(b.size);
+  std::vector<double> hp(D), hhp(D);
+  SpSm pm{env12, b};
+  for (size_t k = 0; k < models.size(); ++k) {
+    PetscInt steps = 0;
+    PetscReal resid = 1;
+    PetscBool conv = PETSC_FALSE;
+    EXPECT_EQ(0, batch.info(PetscInt(k), steps, resid, conv));
+    EXPECT_TRUE(conv);
+    EXPECT_GT(steps, 0);
+    // the per-realisation solve of the reference's loop
+    EPS solver;
+    PetscInt nconv = 0;
+    ASSERT_EQ(0, Solver::SolverInit(solver, models[k]->hamilt, 1));
+    ASSERT_EQ(0, EPSSetTolerances(solver, 1e-12, PETSC_DEFAULT));
+    ASSERT_EQ(0, Solver::solve(env12, solver, nconv));
+    ASSERT_GE(nconv, 1);
+    PetscScalar ek = 0;
+    ASSERT_EQ(0, EPSGetEigenpair(solver, 0, &ek, NULL, NULL, NULL));
+    EXPECT_NEAR(0.0, (e0[k] - ek) / std::fabs(ek), 1e-9);
+    EXPECT_EQ(0, Solver::SolverClean(solver));
+    // ground state: unit norm, Rayleigh quotient E0
+    ASSERT_EQ(0, batch.ground_state(PetscInt(k), psi));
+    ASSERT_EQ(0, MatMult(models[k]->hamilt, psi, hpsi));
+    ASSERT_EQ(0, VecCopyToHost(psi, hp.data()));
+    ASSERT_EQ(0, VecCopyToHost(hpsi, hhp.data()));
+    double nrm = 0.0, rq = 0.0;
+    for (size_t r = 0; r < D; ++r) {
+      nrm += hp[r] * hp[r];
+      rq += hp[r] * hhp[r];
+    }
+    EXPECT_NEAR(1.0, nrm, 1e-12);
+    EXPECT_NEAR(0.0, (rq - ek) / std::fabs(ek), 1e-9);
+    // <S^z_i> = <S^+_i S^-_i> - 1/2, and Euler's identity
+    const double* ok = obs.data() + k * LSQED_BATCH_OBS_STRIDE;
+    double euler = J1 * ok[34] + D1 * ok[33] + J2 * ok[36] + D2 * ok[35];
+    for (int i = 0; i < 12; ++i) {
+      EXPECT_NEAR(pm.ExpectVal(env12, i, i, psi) - 0.5, ok[i], 1e-9);
+      euler += fields[k][size_t(i)] * ok[i];
+    }
+    EXPECT_NEAR(1.0, ok[32], 1e-9);
+    EXPECT_NEAR(0.0, (euler - ek) / std::fabs(ek), 1e-9);
+  }
+  EXPECT_EQ(0, VecDestroy(&psi));
+  EXPECT_EQ(0, VecDestroy(&hpsi));
+}
+
 /* tests/unit_tests/phys/test_dsf1.cpp (Sqz) and test_dsf2.cpp (Sqp). */
 TEST(DSF, chain1D_Sqz) { dsf_case(false); }
 TEST(DSF, chain1D_Sqp) { dsf_case(true); }
