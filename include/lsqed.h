@@ -86,6 +86,13 @@ int lsqed_ham_nup(const lsqed_ham* h);
 /* Which build calls are in effect (LSQED_BUILD_* bits). */
 int lsqed_ham_set_build(lsqed_ham* h, int flags);
 
+/* The model as recorded: couplings (Hamiltonian<L>::hamilt_J1/D1/J2/D2,
+ * include/hamiltonian.hpp:271-295), build flags and bond counts per kind.
+ * Any output may be NULL.  Used by host code that formats the operator
+ * (MatView) from the handle alone. */
+int lsqed_ham_get_params(const lsqed_ham* h, double* J1, double* D1, double* J2, double* D2, int* flags, int* n_nn,
+                         int* n_nnn);
+
 /* Hamiltonian<L>::build_disorder(env, hi) (tmpl:97-125): records the N
  * longitudinal fields (host array, copied) and sets LSQED_BUILD_DISORDER;
  * the diagonal becomes D1/4*c_nn + D2/4*c_nnn + 1/2*sum_i(+-h_i).

@@ -54,6 +54,7 @@ SIGNATURES = {
     "lsqed_ham_nspins": (_i32, [_vp]),
     "lsqed_ham_nup": (_i32, [_vp]),
     "lsqed_ham_set_build": (_i32, [_vp, _i32]),
+    "lsqed_ham_get_params": (_i32, [_vp, _P(_dbl), _P(_dbl), _P(_dbl), _P(_dbl), _P(_i32), _P(_i32), _P(_i32)]),
     "lsqed_ham_set_fields": (_i32, [_vp, _vp]),
     "lsqed_hv": (_i32, [_vp, _vp, _vp, _vp]),
     "lsqed_ham_set_hv_kernel": (_i32, [_vp, _i32]),

@@ -456,6 +456,19 @@ int64_t lsqed_ham_dim(const lsqed_ham* h) { return h ? h->D : 0; }
 int lsqed_ham_nspins(const lsqed_ham* h) { return h ? h->N : 0; }
 int lsqed_ham_nup(const lsqed_ham* h) { return h ? h->nup : 0; }
 
+int lsqed_ham_get_params(const lsqed_ham* h, double* J1, double* D1, double* J2, double* D2, int* flags, int* n_nn,
+                         int* n_nnn) {
+  if (!h) return fail(LSQED_ERR_ARG, "null handle");
+  if (J1) *J1 = h->J[0];
+  if (D1) *D1 = h->Dz[0];
+  if (J2) *J2 = h->J[1];
+  if (D2) *D2 = h->Dz[1];
+  if (flags) *flags = h->build;
+  if (n_nn) *n_nn = (int)h->bi[0].size();
+  if (n_nnn) *n_nnn = (int)h->bi[1].size();
+  return LSQED_OK;
+}
+
 int lsqed_ham_set_build(lsqed_ham* h, int flags) {
   if (!h) return fail(LSQED_ERR_ARG, "null handle");
   if (flags & ~(LSQED_BUILD_DIAG | LSQED_BUILD_OFFDIAG | LSQED_BUILD_DISORDER))

@@ -34,6 +34,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <memory>
@@ -386,6 +388,158 @@ inline PetscErrorCode MatGetDiagonal(Mat mat, Vec d) {
   if (!mat || !d || d->n != lsqed_ham_dim(mat)) return LSQED_ERR_ARG;
   ++d->state;
   return lsqed_diag(mat, d->array, nullptr);
+}
+
+/* ------------------------------------------------------------- ASCII views */
+/* PETSc's default ASCII viewer for the dumps the reference's tests diff
+ * (MatView / VecView, which tests/unit_tests/compare.sh diffs against the
+ * .ref files of tests/unit_tests/reference_data).  `nprocs` is the number of ranks whose layout the
+ * text shows ("N MPI processes", one "Process [k]" block per rank of a
+ * vector, the row split of src/basis.cpp:78-80); the library itself runs on
+ * one process. */
+struct _p_PetscViewer {
+  FILE* fp = nullptr;
+  int nprocs = 1;
+};
+using PetscViewer = _p_PetscViewer*;
+
+inline PetscErrorCode PetscViewerASCIICreate(FILE* fp, int nprocs, PetscViewer* v) {
+  if (!fp || !v || nprocs < 1) return LSQED_ERR_ARG;
+  *v = new _p_PetscViewer;
+  (*v)->fp = fp;
+  (*v)->nprocs = nprocs;
+  return 0;
+}
+inline PetscErrorCode PetscViewerDestroy(PetscViewer* v) {
+  if (v && *v) {
+    delete *v;
+    *v = nullptr;
+  }
+  return 0;
+}
+/* Standard output in the layout of `nprocs` ranks. */
+inline PetscViewer PetscViewerStdout(int nprocs = 1) {
+  static _p_PetscViewer v[64];
+  const int p = nprocs < 1 ? 1 : (nprocs > 64 ? 64 : nprocs);
+  v[p - 1].fp = stdout;
+  v[p - 1].nprocs = p;
+  return &v[p - 1];
+}
+#define PETSC_VIEWER_STDOUT_WORLD (::lsqed::PetscViewerStdout(1))
+
+namespace detail {
+/* PETSc's ASCII scalar format: %g, with a '.' appended to integral output. */
+inline std::string petsc_g(double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%g", v);
+  std::string s(buf);
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".";
+  return s;
+}
+inline void ownership(PetscInt size, int nprocs, int rank, PetscInt& local, PetscInt& start) {
+  const PetscInt rest = size % nprocs;
+  local = size / nprocs + (rest > rank ? 1 : 0);
+  start = rank * local + (rest <= rank ? rest : 0);
+}
+}  // namespace detail
+
+/* VecView of a vector of the sector: a sequential vector on one rank, an
+ * MPI vector with one block per rank otherwise. */
+inline PetscErrorCode VecView(Vec v, PetscViewer viewer) {
+  if (!v || !viewer || !viewer->fp) return LSQED_ERR_ARG;
+  std::vector<double> h(size_t(v->n));
+  if (v->n > 0) {
+    if (PetscErrorCode e = VecCopyToHost(v, h.data())) return e;
+  }
+  const int P = viewer->nprocs;
+  std::fprintf(viewer->fp, "Vec Object: %d MPI processes\n  type: %s\n", P, P == 1 ? "seq" : "mpi");
+  for (int r = 0; r < P; ++r) {
+    PetscInt n = 0, s = 0;
+    detail::ownership(v->n, P, r, n, s);
+    if (P > 1) std::fprintf(viewer->fp, "Process [%d]\n", r);
+    for (PetscInt i = s; i < s + n; ++i) std::fprintf(viewer->fp, "%s\n", detail::petsc_g(h[size_t(i)]).c_str());
+  }
+  return 0;
+}
+
+/* MatView of the operator as the reference assembles it
+ * (include/hamiltonian.tmpl_func.hpp:97-344): per row, J_k/2 for every
+ * antiparallel bond of kind k with |J_k| >= 1e-14 (duplicate columns
+ * summed), and the diagonal where build_disorder set it (every row) or
+ * build_diag added a nonzero bond count of a kind with |D_k| >= 1e-14.
+ * Columns come from the device column export, diagonal values from the
+ * device diagonal. */
+inline PetscErrorCode MatView(Mat mat, PetscViewer viewer) {
+  if (!mat || !viewer || !viewer->fp) return LSQED_ERR_ARG;
+  double J[2], Dz[2];
+  int flags = 0, nb[2] = {0, 0};
+  if (int rc = lsqed_ham_get_params(mat, &J[0], &Dz[0], &J[1], &Dz[1], &flags, &nb[0], &nb[1])) return rc;
+  const PetscInt D = lsqed_ham_dim(mat);
+  std::vector<double> diag(static_cast<size_t>(D));
+  {
+    Vec d = nullptr;
+    if (PetscErrorCode e = VecCreateDevice(D, &d)) return e;
+    PetscErrorCode e = MatGetDiagonal(mat, d);
+    if (!e) e = VecCopyToHost(d, diag.data());
+    VecDestroy(&d);
+    if (e) return e;
+  }
+  std::fprintf(viewer->fp, "Mat Object: %d MPI processes\n  type: mpiaij\n", viewer->nprocs);
+  const PetscInt block = 65536;
+  const int maxc[2] = {std::max(nb[0], 1), std::max(nb[1], 1)};
+  int64_t* d_cols = nullptr;
+  int32_t* d_ncol = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d_cols), size_t(block) * size_t(std::max(maxc[0], maxc[1])) * 8) !=
+          hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&d_ncol), size_t(block) * 4) != hipSuccess) {
+    if (d_cols) (void)hipFree(d_cols);
+    return LSQED_ERR_OOM;
+  }
+  std::vector<int64_t> cols[2];
+  std::vector<int32_t> ncol[2];
+  PetscErrorCode err = 0;
+  for (PetscInt r0 = 0; r0 < D && !err; r0 += block) {
+    const PetscInt nr = std::min(block, D - r0);
+    for (int k = 0; k < 2 && !err; ++k) {
+      cols[k].assign(size_t(nr) * size_t(maxc[k]), -1);
+      ncol[k].assign(size_t(nr), 0);
+      err = lsqed_export_cols(mat, k, r0, nr, maxc[k], d_cols, d_ncol, nullptr);
+      if (!err && hipMemcpy(cols[k].data(), d_cols, cols[k].size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        err = LSQED_ERR_HIP;
+      if (!err && hipMemcpy(ncol[k].data(), d_ncol, ncol[k].size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        err = LSQED_ERR_HIP;
+    }
+    for (PetscInt r = 0; r < nr && !err; ++r) {
+      std::vector<std::pair<int64_t, double>> ent;
+      auto add = [&](int64_t c, double v) {
+        for (auto& e : ent)
+          if (e.first == c) {
+            e.second += v;
+            return;
+          }
+        ent.emplace_back(c, v);
+      };
+      if (flags & LSQED_BUILD_OFFDIAG)
+        for (int k = 0; k < 2; ++k) {
+          if (std::fabs(J[k]) < 1e-14) continue;
+          for (int c = 0; c < ncol[k][size_t(r)]; ++c) add(cols[k][size_t(r) * size_t(maxc[k]) + size_t(c)], J[k] * 0.5);
+        }
+      bool present = false;
+      if (flags & LSQED_BUILD_DISORDER)
+        present = true;
+      else if (flags & LSQED_BUILD_DIAG)
+        for (int k = 0; k < 2; ++k)
+          present = present || (std::fabs(Dz[k]) >= 1e-14 && nb[k] - 2 * ncol[k][size_t(r)] != 0);
+      if (present) add(r0 + r, diag[size_t(r0 + r)]);
+      std::sort(ent.begin(), ent.end());
+      std::fprintf(viewer->fp, "row %lld:", (long long)(r0 + r));
+      for (const auto& e : ent) std::fprintf(viewer->fp, " (%lld, %s) ", (long long)e.first, detail::petsc_g(e.second).c_str());
+      std::fprintf(viewer->fp, "\n");
+    }
+  }
+  (void)hipFree(d_cols);
+  (void)hipFree(d_ncol);
+  return err;
 }
 
 /* ------------------------------------------------------------ Hamiltonian */

@@ -41,3 +41,12 @@ def test_cpp_host_gpu(tmp_path):
     assert "[  FAILED  ]" not in out
     assert os.path.exists(tmp_path / "dsf1" / "dsf_d0.dat")
     assert os.path.exists(tmp_path / "dsf2" / "dsf_d0.dat")
+
+
+@pytest.mark.gpu
+def test_cpp_host_dumps(tmp_path):
+    """The 31 build_diag / build_off_diag / build_disorder dumps of the
+    reference's tests, written by the C++ mirror's MatView / VecView, equal
+    the reference's .ref files character for character."""
+    out = _run("test_host_dumps", tmp_path, 110)
+    assert "[  FAILED  ]" not in out
