@@ -35,6 +35,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -62,6 +63,10 @@ inline PetscReal PetscRealPart(PetscScalar x) { return x; }
 inline PetscReal PetscImaginaryPart(PetscScalar) { return 0.0; }
 
 namespace detail {
+inline uint64_t next_state() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
 inline void check(int rc) {
   if (rc != LSQED_OK) throw std::runtime_error(lsqed_last_error());
 }
@@ -292,9 +297,11 @@ class Striped : public Sublattice<L> {
 /* ------------------------------------------------------------ Mat and Vec */
 using Mat = lsqed_ham*;
 
-/* A vector of the sector in device memory.  `state` counts modifications
- * (PETSc's object state): every call here that writes a Vec increments it,
- * and code that writes through VecGetArray must call VecRestoreArray. */
+/* A vector of the sector in device memory.  `state` marks modifications
+ * (PETSc's object state): every call here that writes a Vec gives it a new
+ * value, unique across all vectors of the process (so a vector created where
+ * a destroyed one lived never repeats its marks), and code that writes
+ * through VecGetArray must call VecRestoreArray. */
 struct _p_Vec {
   double* array = nullptr;
   PetscInt n = 0;
@@ -310,6 +317,7 @@ inline PetscErrorCode VecCreateDevice(PetscInt n, Vec* v) {
   Vec x = new _p_Vec;
   x->array = a;
   x->n = n;
+  x->state = detail::next_state();
   *v = x;
   return 0;
 }
@@ -350,7 +358,7 @@ inline PetscErrorCode VecGetArray(Vec v, PetscScalar** a) {
 }
 inline PetscErrorCode VecRestoreArray(Vec v, PetscScalar** a) {
   if (!v) return LSQED_ERR_ARG;
-  ++v->state;
+  v->state = detail::next_state();
   if (a) *a = nullptr;
   return 0;
 }
@@ -362,7 +370,7 @@ inline PetscErrorCode VecCopyToHost(Vec v, PetscScalar* host) {
 }
 inline PetscErrorCode VecCopyFromHost(Vec v, const PetscScalar* host) {
   if (!v || !host) return LSQED_ERR_ARG;
-  ++v->state;
+  v->state = detail::next_state();
   return detail::hip_status(hipMemcpy(v->array, host, size_t(v->n) * sizeof(double), hipMemcpyHostToDevice));
 }
 
@@ -379,14 +387,14 @@ inline PetscErrorCode MatMult(Mat mat, Vec x, Vec y) {
   if (!mat || !x || !y || x == y) return LSQED_ERR_ARG;
   const PetscInt D = lsqed_ham_dim(mat);
   if (x->n != D || y->n != D) return LSQED_ERR_ARG;
-  ++y->state;
+  y->state = detail::next_state();
   return lsqed_hv(mat, x->array, y->array, nullptr);
 }
 
 /* The stored diagonal; entries the reference never sets read 0. */
 inline PetscErrorCode MatGetDiagonal(Mat mat, Vec d) {
   if (!mat || !d || d->n != lsqed_ham_dim(mat)) return LSQED_ERR_ARG;
-  ++d->state;
+  d->state = detail::next_state();
   return lsqed_diag(mat, d->array, nullptr);
 }
 
@@ -686,9 +694,9 @@ inline PetscErrorCode EPSGetEigenpair(EPS s, PetscInt i, PetscScalar* kr, PetscS
   if ((xr && xr->n != D) || (xi && xi->n != D)) return LSQED_ERR_ARG;
   double lambda = 0.0, err = 0.0;
   if (int rc = lsqed_eps_solution(s->eps, int(i), &lambda, xr ? xr->array : nullptr, &err, nullptr)) return rc;
-  if (xr) ++xr->state;
+  if (xr) xr->state = detail::next_state();
   if (xi) {
-    ++xi->state;
+    xi->state = detail::next_state();
     if (hipMemset(xi->array, 0, size_t(D) * sizeof(double)) != hipSuccess) return LSQED_ERR_HIP;
   }
   if (kr) *kr = lambda;
@@ -1023,7 +1031,7 @@ class BatchSolver {
    * its sector (MatCreateVecs on that model). */
   PetscErrorCode ground_state(PetscInt k, Vec& state) {
     if (!state || k < 0 || k >= size() || state->n != lsqed_ham_dim(hams[size_t(k)])) return LSQED_ERR_ARG;
-    ++state->state;
+    state->state = detail::next_state();
     return lsqed_batch_ground_state(batch, int(k), state->array, nullptr);
   }
   /* dE0/dparameter of every model from its ground state (needs

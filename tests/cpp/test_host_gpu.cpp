@@ -303,6 +303,43 @@ TEST(SzSzCorrelation, chain1D) {
   EXPECT_EQ(0, VecDestroy(&xr));
 }
 
+/* The all-pairs pass behind ExpectVal is reused only while the vector is
+ * unchanged: writing another eigenvector into the same Vec, or into a new
+ * Vec where a destroyed one lived, gives that state's correlators. */
+TEST(SzSzCorrelation, CacheFollowsTheVector) {
+  env.nspins = 8;
+  Basis b{env, 0};
+  chain1D lat{env};
+  Hamiltonian<chain1D> h{env, b, lat, 1.0, 1.0, 0.0, 0.0};
+  ASSERT_EQ(0, h.build_diag(env));
+  ASSERT_EQ(0, h.build_off_diag(env));
+  EPS solver;
+  PetscInt nconv = 0;
+  ASSERT_EQ(0, Solver::SolverInit(solver, h.hamilt, 3));
+  ASSERT_EQ(0, EPSSetTolerances(solver, 1e-12, PETSC_DEFAULT));
+  ASSERT_EQ(0, Solver::solve(env, solver, nconv));
+  ASSERT_GE(nconv, 2);
+  SzSz corr{env, b};
+  PetscScalar e = 0;
+  PetscReal err = 1;
+  Vec x = nullptr;
+  ASSERT_EQ(0, MatCreateVecs(h.hamilt, NULL, &x));
+  ASSERT_EQ(0, Solver::solution(solver, 0, e, x, err));
+  const double c0 = corr.ExpectVal(env, 0, 1, x);
+  EXPECT_NEAR(-0.15212889203236324, c0, 5e-10);  // test_szsz1.cpp:67
+  ASSERT_EQ(0, Solver::solution(solver, 1, e, x, err));  // the first excited state, same Vec
+  const double c1 = corr.ExpectVal(env, 0, 1, x);
+  EXPECT_GT(std::fabs(c1 - c0), 1e-3);
+  for (int round = 0; round < 4; ++round) {
+    ASSERT_EQ(0, VecDestroy(&x));
+    ASSERT_EQ(0, MatCreateVecs(h.hamilt, NULL, &x));
+    ASSERT_EQ(0, Solver::solution(solver, round % 2, e, x, err));
+    EXPECT_NEAR(round % 2 ? c1 : c0, corr.ExpectVal(env, 0, 1, x), 1e-12);
+  }
+  EXPECT_EQ(0, VecDestroy(&x));
+  EXPECT_EQ(0, Solver::SolverClean(solver));
+}
+
 /* tests/unit_tests/correlation/SzSz/test_szsz2.cpp */
 TEST(SzSzCorrelation, square2D_16spins) {
   env.nspins = 16;
