@@ -124,5 +124,18 @@ def stream_ptr(stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
+_EPOCH = [0]
+
+
 def ptr(t) -> C.c_void_p:
+    """Device address of a tensor handed to the library.  Each hand-over
+    advances the write epoch: the library may write there, which torch's
+    version counter does not see."""
+    _EPOCH[0] += 1
     return C.c_void_p(t.data_ptr())
+
+
+def write_epoch() -> int:
+    """Number of tensor hand-overs so far (a cache of results computed from a
+    tensor is valid while this and the tensor's version are unchanged)."""
+    return _EPOCH[0]

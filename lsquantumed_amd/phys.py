@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from ._lib import check, lib, ptr, stream_ptr
+from ._lib import check, lib, ptr, stream_ptr, write_epoch
 from .solver import Solver
 
 
@@ -47,10 +47,15 @@ class _Corr:
         self._cache = None
 
     def ExpectVal(self, env, spin_i, spin_j, state, h=None):
+        """One all-pairs device pass, reused while the state is unchanged:
+        same tensor, same torch version and no library call with a tensor
+        since (the library may have written into it, e.g. Solver.solution)."""
         hh = h or self.h
-        if self._cache is None or self._cache[0] is not state:
-            self._cache = (state, correlations(hh, state))
-        return float(self._cache[1][self.key][spin_i, spin_j])
+        key = (state.data_ptr(), state._version, write_epoch())
+        if self._cache is None or self._cache[0] != key or self._cache[1] is not state:
+            corr = correlations(hh, state)
+            self._cache = ((state.data_ptr(), state._version, write_epoch()), state, corr)
+        return float(self._cache[2][self.key][spin_i, spin_j])
 
 
 class SzSz(_Corr):

@@ -278,3 +278,27 @@ def test_square4x8_ground_state_sum_rules(O, torch):
     assert np.max(np.abs(S.sum(axis=1))) <= 1e-8
     assert abs(obs[OBS_ZZ_NN] + obs[OBS_FLIP_NN] - nn) <= 1e-10 * abs(nn)
     assert abs(obs[OBS_ZZ_NNN] + obs[OBS_FLIP_NNN] - nnn) <= 1e-10 * max(abs(nnn), 1.0)
+
+
+def test_expectval_cache_follows_the_state(torch):
+    """ExpectVal reuses one all-pairs pass only while the state is unchanged:
+    a new eigenvector written into the same tensor by Solver.solution, or an
+    in-place torch copy, gives that state's correlators (test_szsz1.cpp:67 for
+    the ground state)."""
+    import lsquantumed_amd as Q
+    env = Q.Environment(8)
+    h = make("chain", 8, 0, 1.0, 1.0, 0.0, 0.0, 0, 0)
+    eps = Q.Solver.SolverInit(h, 3, tol=1e-12)
+    assert Q.Solver.solve(env, eps) >= 2
+    v = torch.empty(h.hamilt_size(), dtype=torch.float64, device="cuda")
+    corr = Q.SzSz(env, h.basis, h)
+    Q.Solver.solution(eps, 0, v)
+    c0 = corr.ExpectVal(env, 0, 1, v)
+    assert abs(c0 - (-0.15212889203236324)) < 5e-10
+    v0 = v.clone()
+    Q.Solver.solution(eps, 1, v)  # same tensor, written by the library
+    c1 = corr.ExpectVal(env, 0, 1, v)
+    assert abs(c1 - c0) > 1e-3
+    v.copy_(v0)  # in-place torch write
+    assert abs(corr.ExpectVal(env, 0, 1, v) - c0) < 1e-12
+    Q.Solver.SolverClean(eps)
