@@ -147,3 +147,96 @@ class Phys:
                 for n in range(nev):
                     f.write("%.13e  %.13e\n" % (eigs[n] - gs, amp[n, i]))
         return np.array(eigs), amp, nconv
+
+
+# --- post-processing of the DSF files (postprocessing/dynamicStep0_preprocess.ipynb) ---
+def read_dsf(path: str):
+    """Reads a dsf_d<k>.dat file written by Phys.DynStructFactor: returns
+    (delta_e[nev], amp[nev, N], gs).  The file holds one block of nev rows
+    (E_n - E_0, <n|S_i|0>) per spin i (phys.tmpl_func.hpp:190-200); the
+    energies are those of the first block (notebook cell 9)."""
+    gs = None
+    blocks = []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#") and line[1:].lstrip().startswith("GS:"):
+                gs = float(line.split(":")[1])
+            elif line.startswith("# spin:"):
+                blocks.append([])
+            elif line.strip() and not line.startswith("#"):
+                e, a = line.split()
+                blocks[-1].append((float(e), float(a)))
+    if not blocks or any(len(b) != len(blocks[0]) for b in blocks):
+        raise ValueError(f"{path}: not a DynStructFactor file")
+    arr = np.array(blocks)  # [N, nev, 2]
+    return arr[0, :, 0].copy(), arr[:, :, 1].T.copy(), gs
+
+
+def site_coords(lx: int, ly: int = 1) -> np.ndarray:
+    """r_i = (i // ly, i % ly): the site numbering of square2D/honeycomb2D
+    (src/lattice.cpp:94) and of the notebook (cell 3); a chain is (N, 1)."""
+    i = np.arange(lx * ly)
+    return np.stack([i // ly, i % ly], 1).astype(float)
+
+
+def q_grid(lx: int, ly: int = 1) -> np.ndarray:
+    """The N reciprocal vectors of the notebook (cell 4): q_k = ((k // ly)
+    2 pi / lx, (k % ly) 2 pi / ly), k = 0 .. N-1."""
+    k = np.arange(lx * ly)
+    return np.stack([(k // ly) * 2.0 * np.pi / lx, (k % ly) * 2.0 * np.pi / ly], 1)
+
+
+def dsf_weights(amp, lx: int, ly: int = 1, q=None) -> np.ndarray:
+    """W[q, n] = |sum_i exp(i q.r_i) a[n, i]|^2 / N (notebook cells 12-13)
+    for amplitudes amp[n, i] = <n|S_i|0>; q defaults to q_grid(lx, ly)."""
+    a = np.asarray(amp, dtype=float)
+    N = lx * ly
+    if a.ndim != 2 or a.shape[1] != N:
+        raise ValueError("amp must be [nev, lx*ly]")
+    qs = q_grid(lx, ly) if q is None else np.atleast_2d(np.asarray(q, dtype=float))
+    phase = np.exp(1j * (qs @ site_coords(lx, ly).T))  # [nq, N]
+    return np.abs(phase @ a.T) ** 2 / N
+
+
+def multiplet_weights(delta_e, W, tol: float = 1e-8):
+    """Sums W[:, n] over degenerate levels (|E_n - E_m| <= tol max(1, |E|)):
+    returns (level energies, summed weights [nq, levels]).  Per-state
+    amplitudes inside a degenerate level depend on the basis the solver
+    picked; the sums do not."""
+    e = np.asarray(delta_e, dtype=float)
+    W = np.atleast_2d(np.asarray(W, dtype=float))
+    levels, cols = [], []
+    start = 0
+    for n in range(1, len(e) + 1):
+        if n == len(e) or abs(e[n] - e[start]) > tol * max(1.0, abs(e[n])):
+            levels.append(e[start])
+            cols.append(W[:, start:n].sum(axis=1))
+            start = n
+    return np.array(levels), np.stack(cols, 1)
+
+
+def dsf_spectrum(delta_e, W, omega, eps: float) -> np.ndarray:
+    """S(q, w) = sum_n W[q, n] L(w - (E_n - E_0)) of one realisation with the
+    notebook's broadening L(x) = eps^2 / (pi (eps^2 + x^2)) (cells 11, 15)."""
+    e = np.asarray(delta_e, dtype=float)
+    w = np.asarray(omega, dtype=float)
+    L = (eps * eps / np.pi) / (eps * eps + (w[:, None] - e[None, :]) ** 2)  # [nw, nev]
+    return np.asarray(W, dtype=float) @ L.T  # [nq, nw]
+
+
+def disorder_averaged_spectrum(paths, lx: int, ly: int = 1, omega=None, eps: float | None = None):
+    """The notebook's S(q, w): per realisation file the q-resolved weights
+    and broadened spectrum, summed, times 2 pi, divided by the number of
+    realisations (cells 9-17).  Defaults: w in [0, 5] on 5001 points,
+    eps = 1 / N.  Returns (omega, S[nq, nw])."""
+    N = lx * ly
+    w = np.linspace(0.0, 5.0, 5001) if omega is None else np.asarray(omega, dtype=float)
+    eps = 1.0 / N if eps is None else eps
+    acc = None
+    for p in paths:
+        de, amp, _ = read_dsf(p)
+        S = dsf_spectrum(de, dsf_weights(amp, lx, ly), w, eps)
+        acc = S if acc is None else acc + S
+    if acc is None:
+        raise ValueError("no realisation files")
+    return w, acc * 2.0 * np.pi / len(paths)

@@ -237,9 +237,12 @@ def test_square4x6_j1j2_nev10(O, torch):
     m2 = Q.Phys.SquaredSubLattMagAF(env, h, Q.AF(lat), psi0)
     amp = Q.Phys.dsf_amplitudes(h, h, psi0, ev)
     coords = np.array([[i // 6, i % 6] for i in range(24)], float)
+    Wq = Q.phys.dsf_weights(amp, 4, 6, q=(np.pi, np.pi))[0]  # the mirror's q transform
     d_dsf = 0.0
     for grp in fx["dsf_pipi"]:
-        W = sum(O.dsf_weight(amp[n], (np.pi, np.pi), coords) for n in grp["states"])
+        W = sum(Wq[n] for n in grp["states"])
+        Wo = sum(O.dsf_weight(amp[n], (np.pi, np.pi), coords) for n in grp["states"])
+        assert abs(W - Wo) < 1e-14
         d_dsf = max(d_dsf, abs(W - grp["weight"]))
     say("max |d SiSj|", d_sisj, "|d m2|", abs(m2 - fx["m2_af"]), "max |d W(pi,pi)|", d_dsf)
     assert d_sisj <= 1e-8 and abs(m2 - fx["m2_af"]) <= 1e-8 and d_dsf <= 1e-8

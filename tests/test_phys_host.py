@@ -1,0 +1,89 @@
+"""CPU checks of the DSF post-processing of the Python host mirror
+(lsquantumed_amd.phys: read_dsf, site_coords, q_grid, dsf_weights,
+multiplet_weights, dsf_spectrum, disorder_averaged_spectrum), which restates
+postprocessing/dynamicStep0_preprocess.ipynb cells 3-4, 9, 11-17 on the
+files Phys::DynStructFactor writes.  Pinned by the reference's DSF fixture
+(tests/unit_tests/reference_data/dsf_d0_1.ref), by closed forms of the
+lattice Fourier sum and of the broadening integral, and by the oracle's
+dsf_weight / multiplets on the same amplitudes."""
+import os
+
+import numpy as np
+
+from test_oracle import _parse_dsf
+
+from conftest import GOLDEN
+
+REF = os.path.join(GOLDEN, "reference_data")
+
+
+def test_read_dsf_matches_the_fixture():
+    from lsquantumed_amd import phys
+    for name in ("dsf_d0_1.ref", "dsf_d0_2.ref"):
+        path = os.path.join(REF, name)
+        de, amp, gs = phys.read_dsf(path)
+        gs_ref, blocks = _parse_dsf(open(path).read())
+        assert gs == gs_ref
+        assert amp.shape == (10, 6) and de.shape == (10,)
+        assert np.array_equal(de, [e for e, _ in blocks[0]])
+        assert np.array_equal(amp, np.array([[blocks[i][n][1] for i in range(6)] for n in range(10)]))
+
+
+def test_weights_closed_forms():
+    """One site: W = 1/N at every q.  Uniform: N at q = 0, 0 elsewhere.
+    Staggered (-1)^(ix+iy) on a 2x4 lattice: N at q = (pi, pi) only, which
+    is index 6 = 1*ly + 2 (a transposed lx/ly would move it)."""
+    from lsquantumed_amd import phys
+    lx, ly = 2, 4
+    N = lx * ly
+    one = np.zeros((1, N))
+    one[0, 5] = 1.0
+    np.testing.assert_allclose(phys.dsf_weights(one, lx, ly)[:, 0], np.full(N, 1.0 / N), rtol=0, atol=1e-15)
+    uni = phys.dsf_weights(np.ones((1, N)), lx, ly)[:, 0]
+    assert abs(uni[0] - N) < 1e-12 and np.max(np.abs(uni[1:])) < 1e-12
+    r = phys.site_coords(lx, ly)
+    stag = phys.dsf_weights(((-1.0) ** (r[:, 0] + r[:, 1]))[None, :], lx, ly)[:, 0]
+    assert abs(stag[6] - N) < 1e-12 and np.max(np.abs(np.delete(stag, 6))) < 1e-12
+    q = phys.q_grid(lx, ly)
+    assert np.allclose(q[6], (np.pi, np.pi))
+
+
+def test_weights_and_multiplets_vs_oracle(O):
+    """Same amplitudes (the fixture's), every q of the 6-site chain: the
+    mirror's W equals the oracle's dsf_weight, and the multiplet sums use the
+    oracle's grouping of the energies."""
+    from lsquantumed_amd import phys
+    de, amp, _ = phys.read_dsf(os.path.join(REF, "dsf_d0_1.ref"))
+    W = phys.dsf_weights(amp, 6, 1)
+    coords = np.array([[i, 0] for i in range(6)], float)
+    for k, q in enumerate(phys.q_grid(6, 1)):
+        for n in range(10):
+            assert abs(W[k, n] - O.dsf_weight(amp[n], q, coords)) < 1e-14
+    levels, Wl = phys.multiplet_weights(de, W)
+    groups = O.multiplets(de)
+    assert len(levels) == len(groups)
+    for j, g in enumerate(groups):
+        assert abs(levels[j] - de[g[0]]) == 0.0
+        np.testing.assert_allclose(Wl[:, j], W[:, g].sum(axis=1), rtol=0, atol=1e-15)
+
+
+def test_spectrum_integral_and_average():
+    """The broadening eps^2 / (pi (eps^2 + x^2)) integrates to
+    (eps / pi)(atan(b / eps) - atan(a / eps)) over [a, b] (eps over the real
+    line), so the integral over w of S(q, w) is that sum over the levels
+    weighted by W[q, n]; two copies of one file average to 2 pi times that
+    file's spectrum."""
+    from lsquantumed_amd import phys
+    path = os.path.join(REF, "dsf_d0_1.ref")
+    de, amp, _ = phys.read_dsf(path)
+    W = phys.dsf_weights(amp, 6, 1)
+    eps = 0.05
+    w = np.linspace(-400.0, 400.0, 1_600_001)
+    S = phys.dsf_spectrum(de, W, w, eps)
+    integ = np.trapezoid(S, w, axis=1)
+    exact = (eps / np.pi) * (np.arctan((w[-1] - de) / eps) - np.arctan((w[0] - de) / eps))
+    np.testing.assert_allclose(integ, W @ exact, rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(integ, eps * W.sum(axis=1), rtol=1e-4)
+    w2, avg = phys.disorder_averaged_spectrum([path, path], 6, 1)
+    one = phys.dsf_spectrum(de, W, w2, 1.0 / 6)
+    np.testing.assert_allclose(avg, 2.0 * np.pi * one, rtol=1e-14, atol=1e-14)
