@@ -627,6 +627,11 @@ class Hamiltonian {
   PetscReal hamilt_D1() const { return D1; }
   PetscReal hamilt_D2() const { return D2; }
   lattice_type hamilt_type() const { return lattice.get_type(); }
+  /* Which build calls were made (LSQED_BUILD_* bits); no counterpart in the
+   * reference, whose matrix holds the built parts. */
+  int build_flags() const { return flags; }
+  Basis& hamilt_basis() const { return basis; }
+  L& hamilt_lattice() const { return lattice; }
 
  private:
   Basis& basis;
@@ -1049,6 +1054,26 @@ class BatchSolver {
 
 /* ------------------------------------------------------------------ Tools */
 namespace Tools {
+class RandomDisorder;
+}
+
+namespace Phys {
+/* The realisation loop of examples/main_disorder.cpp:82-120 with the
+ * ground states solved in batches: for every realisation k of `rd` not yet
+ * on disk (resume: <data.path>/dsf_d<k>.dat exists), a copy of `h` (same
+ * lattice, couplings and build calls) gets the fields rd.hi_init(k); the
+ * pending realisations' ground states come from BatchSolver (batch_size at
+ * a time), then DynStructFactor<L, SQ> writes each realisation's file with
+ * data.h1 the target-sector model (nullptr: the realisation's own model; Sqp
+ * needs it) carrying the same fields.  Returns {k: E0} for the realisations
+ * computed in this call. */
+template <typename L, typename SQ>
+PetscErrorCode DisorderAverage(Environment& env, Hamiltonian<L>& h, Tools::RandomDisorder& rd, DSF_data<L> data,
+                               std::vector<std::pair<PetscInt, PetscScalar>>& done, PetscBool resume = PETSC_TRUE,
+                               PetscInt batch_size = 256, PetscReal tol = 1e-12);
+}  // namespace Phys
+
+namespace Tools {
 
 /* Tools::RandomDisorder(nspins, reprod, min, max, rep)
  * (src/tools/random_disorder.cpp:9-59): one seed in [0, 1000] per
@@ -1087,6 +1112,81 @@ class RandomDisorder {
 };
 
 }  // namespace Tools
+
+namespace Phys {
+template <typename L, typename SQ>
+PetscErrorCode DisorderAverage(Environment& env, Hamiltonian<L>& h, Tools::RandomDisorder& rd, DSF_data<L> data,
+                               std::vector<std::pair<PetscInt, PetscScalar>>& done, PetscBool resume,
+                               PetscInt batch_size, PetscReal tol) {
+  done.clear();
+  if (batch_size <= 0 || !data.path || (std::is_same<SQ, Sqp>::value && !data.h1)) return LSQED_ERR_ARG;
+  Hamiltonian<L>* target = data.h1;
+  std::vector<PetscInt> pending;
+  for (PetscInt k = 0; k < rd.rep; ++k) {
+    const std::string fn = std::string(data.path) + "/dsf_d" + std::to_string(k) + ".dat";
+    FILE* fp = resume ? std::fopen(fn.c_str(), "r") : nullptr;
+    if (fp)
+      std::fclose(fp);
+    else
+      pending.push_back(k);
+  }
+  auto make = [&](Hamiltonian<L>& like, const std::vector<PetscReal>& hi) {
+    auto m = std::unique_ptr<Hamiltonian<L>>(new Hamiltonian<L>{env, like.hamilt_basis(), like.hamilt_lattice(),
+                                                                like.hamilt_J1(), like.hamilt_D1(), like.hamilt_J2(),
+                                                                like.hamilt_D2()});
+    PetscErrorCode e = 0;
+    if (like.build_flags() & LSQED_BUILD_DIAG) e = m->build_diag(env);
+    if (!e && (like.build_flags() & LSQED_BUILD_OFFDIAG)) e = m->build_off_diag(env);
+    if (!e) e = m->build_disorder(env, hi);
+    if (e) throw std::runtime_error(lsqed_last_error());
+    return m;
+  };
+  for (size_t c0 = 0; c0 < pending.size(); c0 += size_t(batch_size)) {
+    const size_t c1 = std::min(pending.size(), c0 + size_t(batch_size));
+    std::vector<std::unique_ptr<Hamiltonian<L>>> models;
+    std::vector<std::vector<PetscReal>> fields;
+    std::vector<Hamiltonian<L>*> ptrs;
+    for (size_t j = c0; j < c1; ++j) {
+      rd.hi_init(pending[j]);
+      fields.push_back(rd.hi);
+      models.push_back(make(h, rd.hi));
+      ptrs.push_back(models.back().get());
+    }
+    typename BatchSolver<L>::Options o;
+    o.tol = tol;
+    o.keep_vectors = PETSC_TRUE;
+    BatchSolver<L> batch{ptrs, o};
+    std::vector<PetscScalar> e0;
+    if (PetscErrorCode e = batch.solve(e0)) return e;
+    for (size_t j = 0; j < ptrs.size(); ++j) {
+      Vec psi = nullptr;
+      if (PetscErrorCode e = MatCreateVecs(ptrs[j]->hamilt, &psi, NULL)) return e;
+      PetscErrorCode e = batch.ground_state(PetscInt(j), psi);
+      std::unique_ptr<Hamiltonian<L>> tgt;
+      if (!e) {
+        DSF_data<L> d = data;
+        d.h0 = ptrs[j];
+        d.b0 = &ptrs[j]->hamilt_basis();
+        if (target) {
+          tgt = make(*target, fields[j]);
+          d.h1 = tgt.get();
+          d.b1 = &target->hamilt_basis();
+        } else {
+          d.h1 = ptrs[j];
+          d.b1 = d.b0;
+        }
+        d.disorder = PETSC_TRUE;
+        d.hi = fields[j].data();
+        e = DynStructFactor<L, SQ>(env, d, e0[j], psi, pending[c0 + j]);
+      }
+      VecDestroy(&psi);
+      if (e) return e;
+      done.emplace_back(pending[c0 + j], e0[j]);
+    }
+  }
+  return 0;
+}
+}  // namespace Phys
 }  // namespace lsqed
 
 #endif  // LSQUANTUMED_HPP

@@ -656,6 +656,87 @@ TEST(BatchSolver, DisorderChains) {
   EXPECT_EQ(0, VecDestroy(&hpsi));
 }
 
+/* examples/main_disorder.cpp's loop through Phys::DisorderAverage: every
+ * realisation's E0 equals the per-realisation Krylov-Schur solve of the
+ * reference's loop, each file carries its E0 and fields, and a second call
+ * with resume computes only the realisation whose file was removed. */
+TEST(DisorderAverage, ChainResume) {
+  Environment env8{_argc, _argv, 8, help};
+  Basis b{env8, 0};
+  Basis b1{env8, 1};
+  chain1D lat{env8};
+  Hamiltonian<chain1D> h{env8, b, lat, 1.0, 1.0, 0.2, 0.2};
+  ASSERT_EQ(0, h.build_diag(env8));
+  ASSERT_EQ(0, h.build_off_diag(env8));
+  Tools::RandomDisorder rd{8, PETSC_TRUE, -1.0, 1.0, 5};
+  mkdir("./dis", 0755);
+  Phys::DSF_data<chain1D> data{};
+  data.b0 = &b;
+  data.b1 = &b;
+  data.lat = &lat;
+  data.nev = 4;
+  data.path = "./dis";
+  std::vector<std::pair<PetscInt, PetscScalar>> done;
+  ASSERT_EQ(0, (Phys::DisorderAverage<chain1D, Sqz>(env8, h, rd, data, done)));
+  ASSERT_EQ(size_t(5), done.size());
+  std::vector<double> e0(5);
+  for (const auto& kv : done) {
+    const PetscInt k = kv.first;
+    e0[size_t(k)] = kv.second;
+    // the reference's loop for this realisation
+    rd.hi_init(k);
+    Hamiltonian<chain1D> hk{env8, b, lat, 1.0, 1.0, 0.2, 0.2};
+    ASSERT_EQ(0, hk.build_diag(env8));
+    ASSERT_EQ(0, hk.build_off_diag(env8));
+    ASSERT_EQ(0, hk.build_disorder(env8, rd.hi));
+    EPS solver;
+    PetscInt nconv = 0;
+    ASSERT_EQ(0, Solver::SolverInit(solver, hk.hamilt, 1));
+    ASSERT_EQ(0, EPSSetTolerances(solver, 1e-12, PETSC_DEFAULT));
+    ASSERT_EQ(0, Solver::solve(env8, solver, nconv));
+    PetscScalar ek = 0;
+    ASSERT_EQ(0, EPSGetEigenpair(solver, 0, &ek, NULL, NULL, NULL));
+    EXPECT_NEAR(0.0, (kv.second - ek) / std::fabs(ek), 1e-10);
+    EXPECT_EQ(0, Solver::SolverClean(solver));
+    DsfFile f;
+    ASSERT_TRUE(read_dsf("./dis/dsf_d" + std::to_string(k) + ".dat", f));
+    ASSERT_EQ(size_t(8), f.rows.size());
+    EXPECT_EQ(size_t(4), f.rows[0].size());
+    double gs_line = 0.0, h0_line = 1e9;
+    for (const std::string& line : f.header) {
+      std::sscanf(line.c_str(), "#      GS:   %lf", &gs_line);
+      std::sscanf(line.c_str(), "# hi[   0]: %lf", &h0_line);
+    }
+    EXPECT_NEAR(kv.second, gs_line, 1e-15 * std::fabs(kv.second) + 1e-16);
+    EXPECT_NEAR(rd.hi[0], h0_line, 1e-16);
+    EXPECT_NEAR(0.0, f.rows[0][0].first, 1e-9);  // E_0 - E_0 of the same sector
+  }
+  // resume: only the removed file is recomputed
+  ASSERT_EQ(0, std::remove("./dis/dsf_d2.dat"));
+  ASSERT_EQ(0, (Phys::DisorderAverage<chain1D, Sqz>(env8, h, rd, data, done)));
+  ASSERT_EQ(size_t(1), done.size());
+  EXPECT_EQ(2, done[0].first);
+  EXPECT_NEAR(0.0, (done[0].second - e0[2]) / std::fabs(e0[2]), 1e-10);
+  // Sqp into the Sz = 1 sector of the same couplings (fields copied per realisation)
+  Hamiltonian<chain1D> h1{env8, b1, lat, 1.0, 1.0, 0.2, 0.2};
+  ASSERT_EQ(0, h1.build_diag(env8));
+  ASSERT_EQ(0, h1.build_off_diag(env8));
+  mkdir("./disp", 0755);
+  Phys::DSF_data<chain1D> dp = data;
+  dp.b1 = &b1;
+  dp.h1 = &h1;
+  dp.path = "./disp";
+  Tools::RandomDisorder rd2{8, PETSC_TRUE, -1.0, 1.0, 2};
+  ASSERT_EQ(0, (Phys::DisorderAverage<chain1D, Sqp>(env8, h, rd2, dp, done)));
+  ASSERT_EQ(size_t(2), done.size());
+  DsfFile fp;
+  ASSERT_TRUE(read_dsf("./disp/dsf_d1.dat", fp));
+  ASSERT_EQ(size_t(8), fp.rows.size());
+  EXPECT_EQ(size_t(4), fp.rows[7].size());
+  dp.h1 = nullptr;  // Sqp needs the target sector
+  EXPECT_TRUE((Phys::DisorderAverage<chain1D, Sqp>(env8, h, rd2, dp, done, PETSC_FALSE)) != 0);
+}
+
 /* tests/unit_tests/phys/test_dsf1.cpp (Sqz) and test_dsf2.cpp (Sqp). */
 TEST(DSF, chain1D_Sqz) { dsf_case(false); }
 TEST(DSF, chain1D_Sqp) { dsf_case(true); }
