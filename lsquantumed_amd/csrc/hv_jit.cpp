@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -423,6 +424,9 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   std::ostringstream o;
   o << "// generated by liblsqed (hv_jit.cpp)\n#define BATCH " << (batch ? 1 : 0) << "\n#define WIDE "
     << (N > 32 ? 1 : 0) << "\n";
+  // occupancy hint of the kernel (waves per SIMD); LSQED_JIT_WAVES_PER_EU overrides the default 4
+  if (const char* w = getenv("LSQED_JIT_WAVES_PER_EU"))
+    if (atoi(w) >= 1 && atoi(w) <= 8) o << "#define WAVES_PER_EU " << atoi(w) << "\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
