@@ -5,7 +5,7 @@ TEST INFRASTRUCTURE ONLY: this module is the checker.  Only ``tests/``,
 import it; the product package ``lsquantumed_amd`` never does.
 
 It wraps ``oracle/liboracle.so`` (built by ``oracle/Makefile`` from
-``oracle/oracle.cpp``, a line-by-line restatement of the reference's
+``oracle/oracle.cpp``, a routine-by-routine restatement of the reference's
 arithmetic) and adds the numpy-side pieces of the reference that are plain
 linear algebra:
 
@@ -18,7 +18,7 @@ linear algebra:
 
 Parity status of every function is listed in ``PARITY`` below; every entry
 is pinned by a reference fixture, a reference test literal, or a closed
-form / brute-force check in ``tests/test_oracle_*.py``.
+form / brute-force check in ``tests/test_oracle.py``.
 """
 from __future__ import annotations
 
