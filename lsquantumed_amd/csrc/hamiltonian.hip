@@ -309,7 +309,7 @@ int hv_launch_grid(lsqed_ham* h) {
   const int k = hv_kind(h);
   if (k >= 2) {
     if (split_plan_create(h) != LSQED_OK) return 0;
-    return split_grid(h);
+    return k == 3 ? jit_grid(h) : split_grid(h);
   }
   return hv_grid(h, (h->D + kHvThreads - 1) / kHvThreads);
 }
@@ -322,7 +322,7 @@ int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, hi
   if (k == 3) {
     int rc = split_plan_create(h);
     if (rc) return rc;
-    return launch_hv_jit(h, x, y, dot_partials, split_grid(h), s);
+    return launch_hv_jit(h, x, y, dot_partials, jit_grid(h), s);
   }
   if (k == 2) return launch_hv_split(h, x, y, dot_partials, 0, s);
   const int64_t ntiles = (h->D + kHvThreads - 1) / kHvThreads;

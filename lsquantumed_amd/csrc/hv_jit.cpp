@@ -29,7 +29,16 @@ namespace lsqed {
 // Device code of the specialised kernel.  The generated prologue defines
 // N, NUP, L, NH, W1, the bond class arrays (LOW*, CR*, HI*) and their counts.
 static const char* kJitBody = R"JIT(
-constexpr int kThreads = 256;
+#ifndef MASKED_GATHER
+#define MASKED_GATHER 0
+#endif
+#ifndef ROWSTAGE
+#define ROWSTAGE 0
+#endif
+// ROWSTAGE: one block of 1024 threads per row, the row's x staged in LDS, so
+// that the low bonds (targets inside the row) read LDS instead of issuing
+// global gathers
+constexpr int kThreads = ROWSTAGE ? 1024 : 256;
 constexpr int kPerLane = 8;
 constexpr int kChunk = kThreads * kPerLane;
 
@@ -85,6 +94,9 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   __shared__ int s_nact, s_cnt0, s_cnt1;
   __shared__ double s_dish;
   __shared__ double s_red[kThreads / 64];
+#if ROWSTAGE
+  __shared__ double s_x[RMAX];
+#endif
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += kThreads) s_RL0[i] = a.RL0[i];
   for (int i = tid; i < 9 * W1; i += kThreads) s_RL1[i] = a.RL1[i];
@@ -176,6 +188,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     }
     __syncthreads();
     const BT B = rfl(s_B);
+#if ROWSTAGE
+    const int Rrow = (task >> 24) ? 0 : R;  // padding items (chunk field 0xff) hold no row
+    for (int e = tid; e < Rrow; e += kThreads) s_x[e] = xv[B + (BT)e];
+    __syncthreads();
+#endif
     const int nact = __builtin_amdgcn_readfirstlane(s_nact);
     const int cnt0 = __builtin_amdgcn_readfirstlane(s_cnt0), cnt1 = __builtin_amdgcn_readfirstlane(s_cnt1);
     BT cb[NH + 1];
@@ -183,9 +200,15 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     for (int q = 0; q < NH; ++q) cb[q] = rfl(s_cbase[q]);
     const unsigned short* Uk = a.U + a.offU[kl];
 #pragma unroll 1
+#if ROWSTAGE
+    for (int k = 0; k < (RMAX + kThreads - 1) / kThreads; ++k) {
+      const int i = tid + k * kThreads;
+      if (i >= Rrow) break;
+#else
     for (int k = 0; k < kPerLane; ++k) {
       const int i = i0 + tid + k * kThreads;
       if (i >= R) break;
+#endif
       const BT r = B + (BT)i;
       double acc = 0.0;
       // high bonds: row h ^ m, same entry; loads issued in batches of 8
@@ -230,8 +253,13 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
             } else {
               i2 = t0 + s_RL1[k0 * W1 + (l2 >> 8)];
             }
+#if MASKED_GATHER
+            v[u] = 0.0;
+            if (on) v[u] = xv[cb[hp] + (BT)i2];
+#else
             const BT idx = on ? cb[hp] + (BT)i2 : r;
             v[u] = xv[idx];
+#endif
           }
         }
 #pragma unroll
@@ -266,8 +294,15 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
               const unsigned int w0 = l2 & 255u;
               i2 = s_RL0[w0] + s_RL1[__builtin_popcount(w0) * W1 + (l2 >> 8)];
             }
+#if ROWSTAGE
+            v[u] = s_x[on ? i2 : (unsigned int)i];
+#elif MASKED_GATHER
+            v[u] = 0.0;
+            if (on) v[u] = xv[B + (BT)i2];
+#else
             const BT idx = on ? B + (BT)i2 : r;
             v[u] = xv[idx];
+#endif
           }
         }
 #pragma unroll
@@ -290,7 +325,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         dis += s_dish;
         d = (dq0_ * (double)c0 + dq1_ * (double)c1) + 0.5 * dis;
       }
+#if ROWSTAGE
+      const double xr = s_x[i];
+#else
       const double xr = xv[r];
+#endif
       const double yr = d * xr + acc;
       yv[r] = yr;
       if (DOT) dotacc += xr * yr;
@@ -374,8 +413,20 @@ struct JitModule {
   hipModule_t mod = nullptr;
   hipFunction_t fn_dot = nullptr, fn_nodot = nullptr;
   bool ok = false;
+  bool rowstage = false;  // one 1024-thread block per row (rows_ord), else 256 threads per item
   std::string log;
 };
+
+// The single-model kernel stages each row of x in LDS (one 1024-thread block
+// per row) unless LSQED_JIT_ROWSTAGE=0.  The low bonds then read LDS instead
+// of issuing global gathers: 37% fewer vector memory instructions on the
+// 4x8 J1-J2 sector, whose H.v was bound by address processing (TA busy 92%);
+// 43.7 -> 41.0 ms there, 779 -> 736 ms on 6x6, 0.185 -> 0.152 ms on 4x6,
+// bit-identical output.
+static bool jit_rowstage() {
+  const char* e = getenv("LSQED_JIT_ROWSTAGE");
+  return !(e && atoi(e) == 0);
+}
 
 static std::mutex g_jit_mu;
 static std::map<std::string, std::shared_ptr<JitModule>> g_jit_cache;
@@ -427,6 +478,10 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   // occupancy hint of the kernel (waves per SIMD); LSQED_JIT_WAVES_PER_EU overrides the default 4
   if (const char* w = getenv("LSQED_JIT_WAVES_PER_EU"))
     if (atoi(w) >= 1 && atoi(w) <= 8) o << "#define WAVES_PER_EU " << atoi(w) << "\n";
+  if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
+    if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
+  if (!batch && jit_rowstage())
+    o << "#define ROWSTAGE 1\nconstexpr int RMAX = " << binom_u64(L, L / 2) << ";\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
@@ -486,6 +541,7 @@ static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch
     return m;
   }
   m->ok = true;
+  m->rowstage = src.find("#define ROWSTAGE 1") != std::string::npos;
   return m;
 }
 
@@ -515,6 +571,16 @@ int jit_prepare(lsqed_ham* h) {
   return LSQED_OK;
 }
 
+// Blocks of one specialised H.v (one per work item, or one per row when the
+// kernel stages rows); also the number of x.y partial sums it writes.
+int jit_grid(lsqed_ham* h) {
+  if (jit_prepare(h) != LSQED_OK) return 0;
+  SplitTables t;
+  if (split_tables(h, t) != LSQED_OK) return 0;
+  const int n = ((JitModule*)h->jit)->rowstage ? t.nrows_ord : t.ntasks_ord;
+  return std::max(8, ((n + 7) / 8) * 8);
+}
+
 int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s) {
   int rc = jit_prepare(h);
   if (rc) return rc;
@@ -532,12 +598,12 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
   a.RL1 = t.RL1;
   a.binom = t.binom;
   a.binom64 = t.binom64;
-  a.tasks = t.tasks_ord;
-  a.ntasks = t.ntasks_ord;
+  a.tasks = m->rowstage ? t.rows_ord : t.tasks_ord;
+  a.ntasks = m->rowstage ? t.nrows_ord : t.ntasks_ord;
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
   void* args[] = {&a};
-  LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, args,
-                                  nullptr));
+  LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, m->rowstage ? 1024 : 256,
+                                  1, 1, 0, s, args, nullptr));
   return LSQED_OK;
 }
 

@@ -65,6 +65,8 @@ struct SplitPlan {
   int ntasks = 0;
   uint32_t* d_tasks_ord = nullptr;  // the same items grouped for cache reuse, one segment per XCD
   int ntasks_ord = 0;               // a multiple of 8 (padding items process no entries)
+  uint32_t* d_rows_ord = nullptr;   // whole rows, same order and padding (row-staged kernel)
+  int nrows_ord = 0;
   size_t binom_n = 0;
 };
 
@@ -284,6 +286,7 @@ void split_plan_destroy(lsqed_ham* h) {
   (void)hipFree(sp->d_binom64);
   (void)hipFree(sp->d_tasks);
   (void)hipFree(sp->d_tasks_ord);
+  (void)hipFree(sp->d_rows_ord);
   delete sp;
   h->split = nullptr;
 }
@@ -301,20 +304,28 @@ void split_plan_destroy(lsqed_ham* h) {
 // sites are picked greedily to add the fewest bonds leaving the group
 // (degree minus bonds to sites already outer).  ko = 0 keeps h order.  The
 // groups serve the memory-side cache, not L2: L2 fills are unchanged
-// (FETCH_SIZE equal within 0.3%), the 4x8 J1-J2 H.v goes from 44.9 to
-// 43.6 ms at ko = 5 (150 MB groups), the 6x6 one from 781.6 to 780.2 ms.
-// LSQED_SPLIT_OUTER (ko) and LSQED_SPLIT_GROUP_MB override the choice.
+// (FETCH_SIZE equal within 0.3%).  With 2048-entry items the 4x8 J1-J2 H.v
+// went from 44.9 to 43.6 ms at ko = 5 (150 MB groups), the 6x6 one from 781.6
+// to 780.2 ms; with whole rows staged in LDS (the default specialised kernel)
+// h order is best (4x8: 40.96 ms at ko = 0, 41.78 at ko = 5), so grouping is
+// off unless LSQED_SPLIT_OUTER (ko) or LSQED_SPLIT_GROUP_MB asks for it.
 static int outer_site_count(const lsqed_ham* h, int NH) {
   if (const char* e = std::getenv("LSQED_SPLIT_OUTER")) return std::max(0, std::min(NH, std::atoi(e)));
-  double mb = 160.0;
-  if (const char* e = std::getenv("LSQED_SPLIT_GROUP_MB")) mb = std::max(1.0, std::atof(e));
+  const char* g = std::getenv("LSQED_SPLIT_GROUP_MB");
+  if (!g) return 0;
+  const double mb = std::max(1.0, std::atof(g));
   const double bytes = 8.0 * (double)h->D;
   int ko = 0;
   while (ko < NH && bytes / std::ldexp(1.0, ko) > mb * 1048576.0) ++ko;
   return ko;
 }
 
-static std::vector<uint32_t> order_tasks(const lsqed_ham* h, int L, int NH, int ko) {
+// Fills `tasks` with the (row, chunk) items and `rows` with the rows (chunk
+// field 0; the row-staged kernel takes a whole row per block), both in group
+// order with each group cut into eighths, one per XCD, padded with items that
+// process no entries (chunk field 0xff).
+static void order_items(const lsqed_ham* h, int L, int NH, int ko, std::vector<uint32_t>& tasks,
+                        std::vector<uint32_t>& rows) {
   std::vector<int> deg(NH, 0);
   std::vector<std::vector<int>> adj(NH, std::vector<int>(NH, 0));
   for (int k = 0; k < 2; ++k)
@@ -346,10 +357,11 @@ static std::vector<uint32_t> order_tasks(const lsqed_ham* h, int L, int NH, int 
     if (!is_outer[q]) inner.push_back(q);
   const int nup = h->nup;
   uint32_t pad = 0xffffffffu;  // an item with a valid row and no entries
-  std::vector<std::vector<uint32_t>> seg(8);
-  std::vector<uint32_t> group;
+  std::vector<std::vector<uint32_t>> seg(8), rseg(8);
+  std::vector<uint32_t> group, grows;
   for (uint32_t o = 0; o < (1u << ko); ++o) {
     group.clear();
+    grows.clear();
     uint32_t ho = 0;
     for (int t = 0; t < ko; ++t)
       if ((o >> t) & 1u) ho |= 1u << outer[t];
@@ -362,15 +374,21 @@ static std::vector<uint32_t> order_tasks(const lsqed_ham* h, int L, int NH, int 
       if (pad == 0xffffffffu) pad = hh | (0xffu << 24);
       const int64_t R = (int64_t)binom_u64(L, kl);
       for (int64_t c = 0; c * kSplitChunk < R; ++c) group.push_back(hh | ((uint32_t)c << 24));
+      grows.push_back(hh);
     }
     if (group.empty()) continue;
-    const size_t per = (group.size() + 7) / 8;
-    for (size_t x = 0; x < 8; ++x)
+    const size_t per = (group.size() + 7) / 8, rper = (grows.size() + 7) / 8;
+    for (size_t x = 0; x < 8; ++x) {
       for (size_t i = x * per; i < (x + 1) * per; ++i) seg[x].push_back(i < group.size() ? group[i] : pad);
+      for (size_t i = x * rper; i < (x + 1) * rper; ++i) rseg[x].push_back(i < grows.size() ? grows[i] : pad);
+    }
   }
-  std::vector<uint32_t> out;
-  for (int x = 0; x < 8; ++x) out.insert(out.end(), seg[x].begin(), seg[x].end());
-  return out;
+  tasks.clear();
+  rows.clear();
+  for (int x = 0; x < 8; ++x) {
+    tasks.insert(tasks.end(), seg[x].begin(), seg[x].end());
+    rows.insert(rows.end(), rseg[x].begin(), rseg[x].end());
+  }
 }
 
 int split_plan_create(lsqed_ham* h) {
@@ -449,8 +467,10 @@ int split_plan_create(lsqed_ham* h) {
   }
   sp->ntasks = (int)tasks.size();
   const int ko = outer_site_count(h, NH);
-  std::vector<uint32_t> tasks_ord = ko > 0 ? order_tasks(h, L, NH, ko) : tasks;
+  std::vector<uint32_t> tasks_ord, rows_ord;
+  order_items(h, L, NH, ko, tasks_ord, rows_ord);
   sp->ntasks_ord = (int)tasks_ord.size();
+  sp->nrows_ord = (int)rows_ord.size();
   p.ntasks = sp->ntasks_ord;
   auto up = [&](void** dst, const void* src, size_t bytes) -> bool {
     if (hipMalloc(dst, std::max<size_t>(bytes, 4)) != hipSuccess) return false;
@@ -460,7 +480,8 @@ int split_plan_create(lsqed_ham* h) {
       !up((void**)&sp->d_RL1, RL1.data(), RL1.size() * 2) || !up((void**)&sp->d_binom, bin.data(), bin.size() * 4) ||
       !up((void**)&sp->d_binom64, bin64.data(), bin64.size() * 8) ||
       !up((void**)&sp->d_tasks, tasks.data(), tasks.size() * 4) ||
-      !up((void**)&sp->d_tasks_ord, tasks_ord.data(), tasks_ord.size() * 4)) {
+      !up((void**)&sp->d_tasks_ord, tasks_ord.data(), tasks_ord.size() * 4) ||
+      !up((void**)&sp->d_rows_ord, rows_ord.data(), rows_ord.size() * 4)) {
     h->split = sp;
     split_plan_destroy(h);
     return fail(LSQED_ERR_HIP, "row-split table upload failed");
@@ -487,6 +508,8 @@ int split_tables(lsqed_ham* h, SplitTables& t) {
   t.ntasks = sp->ntasks;
   t.tasks_ord = sp->d_tasks_ord;
   t.ntasks_ord = sp->ntasks_ord;
+  t.rows_ord = sp->d_rows_ord;
+  t.nrows_ord = sp->nrows_ord;
   t.offU = sp->p.offU;
   return LSQED_OK;
 }

@@ -132,6 +132,8 @@ struct SplitTables {
   int ntasks;
   const uint32_t* tasks_ord;  // the items of one H.v in cache-reuse order (split_grid blocks)
   int ntasks_ord;
+  const uint32_t* rows_ord;  // whole rows in the same order (one block per row)
+  int nrows_ord;
   const int* offU;
 };
 bool split_supported(const lsqed_ham* h);
@@ -147,6 +149,7 @@ bool split_jit_fits(const lsqed_ham* h);
 bool jit_enabled();
 int jit_prepare(lsqed_ham* h);
 int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
+int jit_grid(lsqed_ham* h);
 
 // Batch variant of the specialised kernel (hv_jit.cpp): problems sharing a
 // bond structure and build mode form a group; one launch covers the group.
