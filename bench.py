@@ -503,6 +503,46 @@ def lanczos_6x6(maxit: int = 600, tol: float = 1e-9):
             "device_gb": round(3 * D * 8 / 1e9, 1)}
 
 
+def hv_only(lx: int, ly: int, reps: int, warmup: int):
+    """H.v alone on the periodic lx x ly square lattice (J1=D1=1, J2=D2=0.5,
+    Sz=0) with two device vectors, timed with HIP events over `reps` launches
+    on the launch stream; x.(Hx) is reported as a checksum.  The row split
+    and block size follow LSQED_SPLIT_L / LSQED_JIT_ROW_THREADS when set."""
+    import torch
+
+    import lsquantumed_amd as Q
+
+    N = lx * ly
+    env = Q.Environment(N)
+    lat = Q.square2D(env, lx, ly)
+    h = Q.Hamiltonian(env, Q.Basis(env, 0), lat, 1.0, 1.0, 0.5, 0.5)
+    h.build_diag(env)
+    h.build_off_diag(env)
+    D = h.hamilt_size()
+    B, _, _ = algorithmic_bytes(N, N // 2, len(lat.nn_bonds[0]) + len(lat.nnn_bonds[0]))
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(D, dtype=torch.float64, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, warmup)):
+        h.matvec(x, y)
+    torch.cuda.synchronize()
+    chk = sum(float(torch.dot(x[i:i + 2**30], y[i:i + 2**30]).item()) for i in range(0, D, 2**30))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        h.matvec(x, y)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    kernel = {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()]
+    return {"workload": f"H.v, periodic square {lx}x{ly} (N={N}), J1=D1=1, J2=D2=0.5, Sz=0, D={D}",
+            "hv_kernel": kernel, "split_low_bits": os.environ.get("LSQED_SPLIT_L", "default"),
+            "row_threads": os.environ.get("LSQED_JIT_ROW_THREADS", "default"), "hv_ms": round(ms, 4),
+            "algorithmic_bytes": B, "gbs": round(B / ms / 1e6, 1), "roofline_frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+            "x_dot_hx": chk}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -519,6 +559,8 @@ def main():
                     help="only the 6x6 (N=36) three-vector Lanczos ground state (218 GB; minutes)")
     ap.add_argument("--krylov-4x8", action="store_true",
                     help="only Krylov-Schur nev=4 ncv=20 on the 4x8 sector (101 GB basis; about a minute)")
+    ap.add_argument("--hv-only", action="store_true",
+                    help="only H.v on the --lx x --ly lattice with two vectors (6x6: 145 GB); one JSON object")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Sz-sector path here is built and measured on a single "
@@ -538,6 +580,9 @@ def main():
         return
     if args.krylov_4x8:
         print(json.dumps({"krylov_schur_4x8": krylov_schur_4x8()}))
+        return
+    if args.hv_only:
+        print(json.dumps({"hv": hv_only(args.lx, args.ly, args.steps, args.warmup)}))
         return
     lx, ly = args.lx, args.ly
     N = lx * ly

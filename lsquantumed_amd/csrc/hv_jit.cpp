@@ -35,10 +35,13 @@ static const char* kJitBody = R"JIT(
 #ifndef ROWSTAGE
 #define ROWSTAGE 0
 #endif
-// ROWSTAGE: one block of 1024 threads per row, the row's x staged in LDS, so
-// that the low bonds (targets inside the row) read LDS instead of issuing
-// global gathers
-constexpr int kThreads = ROWSTAGE ? 1024 : 256;
+// ROWSTAGE: blocks of ROW_THREADS threads take whole rows, staging each
+// row's x in LDS, so that the low bonds (targets inside the row) read LDS
+// instead of issuing global gathers
+#ifndef ROW_THREADS
+#define ROW_THREADS 1024
+#endif
+constexpr int kThreads = ROWSTAGE ? ROW_THREADS : 256;
 constexpr int kPerLane = 8;
 constexpr int kChunk = kThreads * kPerLane;
 
@@ -81,18 +84,25 @@ struct JitArgs {
 #endif
 };
 
+#if ROWSTAGE
+constexpr int kStage = (RMAX + kThreads - 1) / kThreads;  // entries of a row per thread
+#endif
+
 template <bool DOT>
 __device__ __forceinline__ void hv_body(const JitArgs& a) {
+  // the row's bond tables; under ROWSTAGE double buffered by row parity, so
+  // that the next row's are built while waves still read the current ones
+  constexpr int NBUF = ROWSTAGE ? 2 : 1;
   __shared__ unsigned short s_RL0[256];
   __shared__ unsigned short s_RL1[9 * W1];
   __shared__ BT s_binom[(N + 1) * (NUP + 1)];
   __shared__ double s_f[N];
-  __shared__ BT s_hbase[NHIGH + 1];
-  __shared__ double s_hcoef[NHIGH + 1];
-  __shared__ BT s_cbase[NH + 1];
+  __shared__ BT s_hbase[NBUF][NHIGH + 1];
+  __shared__ double s_hcoef[NBUF][NHIGH + 1];
+  __shared__ BT s_cbase[NBUF][NH + 1];
   __shared__ BT s_B;
-  __shared__ int s_nact, s_cnt0, s_cnt1;
-  __shared__ double s_dish;
+  __shared__ int s_nact[NBUF], s_cnt0[NBUF], s_cnt1[NBUF];
+  __shared__ double s_dish[NBUF];
   __shared__ double s_red[kThreads / 64];
 #if ROWSTAGE
   __shared__ double s_x[RMAX];
@@ -122,10 +132,10 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   };
   constexpr int flip[2] = {FLIP0, FLIP1};
   const long long G = gridDim.x;
-  const long long lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
 #if BATCH
   // tasks t = (problem of the group) * ntasks + (row-split task), dealt
   // cyclically so that all blocks work on neighbouring problems together
+  const long long lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
   const long long ntot = (long long)a.nprob * a.ntasks;
   int loaded = -1;
   for (long long t = lb; t < ntot; t += G) {
@@ -143,10 +153,28 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       loaded = prob;
     }
     double dotacc = 0.0;
+    constexpr int pb = 0;
     const unsigned int task = a.tasks[st];
+#elif ROWSTAGE
+  // The row list holds eight equal segments, one per XCD; block b runs on
+  // XCD b % 8 (the dispatcher deals blocks round-robin over the XCDs) and
+  // takes rows b / 8, b / 8 + P, ... of its segment (P blocks per XCD; one
+  // row per block when the grid covers the list), so the blocks of one XCD
+  // work on adjacent rows at a time and meet in its L2.
+  const int xcd = blockIdx.x & 7, P = (int)(G >> 3), seg = a.ntasks >> 3;
+  double dotacc = 0.0;
+  int pb = 0;
+  for (int t = (int)(blockIdx.x >> 3); t < seg; t += P, pb ^= 1) {
+    const double* __restrict__ xv = a.x;
+    double* __restrict__ yv = a.y;
+    constexpr double jh_[2] = {JH0, JH1};
+    constexpr double dq0_ = DQ0, dq1_ = DQ1;
+    const unsigned int task = a.tasks[xcd * seg + t];
 #else
+  const long long lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
   const long long ta = lb * a.ntasks / G, tb = (lb + 1) * a.ntasks / G;
   double dotacc = 0.0;
+  constexpr int pb = 0;
   for (long long t = ta; t < tb; ++t) {
     const double* __restrict__ xv = a.x;
     double* __restrict__ yv = a.y;
@@ -156,54 +184,108 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #endif
     const unsigned int h = task & 0xffffffu;
     const int kl = NUP - __builtin_popcount(h);
+#if ROWSTAGE
+    const int R = (task >> 24) ? 0 : (int)C(L, kl);  // padding items (chunk field 0xff) hold no row
+    const int i0 = 0;
+    // every thread computes the row base and issues its staging loads now;
+    // wave 0 (high bonds, compacted with a ballot) and wave 1 (cross rows,
+    // field sum) fill this row's table buffer meanwhile
+    const BT B = rfl(rowbase(h));
+    double stage[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = tid + u * kThreads < R ? tid + u * kThreads : R - 1;
+      stage[u] = 0.0;
+      if (u * kThreads < R) stage[u] = xv[B + (BT)e];
+    }
+    if (tid < 64) {
+      int nact = 0, c0 = 0, c1 = 0;
+#pragma unroll
+      for (int g = 0; g < NHIGH; g += 64) {  // lane tid: high bond g + tid
+        const int b = g + tid;
+        bool anti = false, act = false;
+        int kind = 0;
+        BT base = 0;
+        if (b < NHIGH) {
+          const unsigned int m = HIM[b];
+          kind = HIK[b];
+          anti = __builtin_popcount(h & m) == 1;
+          act = anti && flip[kind];
+          if (act) base = rowbase(h ^ m);
+        }
+        const unsigned long long bal = __ballot(act);
+        const int pos = nact + __popcll(bal & ((1ull << tid) - 1ull));
+        if (act) {
+          s_hbase[pb][pos] = base;
+          s_hcoef[pb][pos] = jh_[kind];
+        }
+        nact += __popcll(bal);
+        c0 += __popcll(__ballot(anti && kind == 0));
+        c1 += __popcll(__ballot(anti && kind == 1));
+      }
+      if (tid == 0) {
+        s_nact[pb] = nact;
+        s_cnt0[pb] = c0;
+        s_cnt1[pb] = c1;
+      }
+    } else if (tid < 64 + NH) {
+      s_cbase[pb][tid - 64] = rowbase(h ^ (1u << (tid - 64)));
+    } else if (tid == 127) {
+      double dh = 0.0;
+      if (DIAG_MODE == 1)
+        for (int q = 0; q < NH; ++q) dh = ((h >> q) & 1u) ? dh + s_f[L + q] : dh - s_f[L + q];
+      s_dish[pb] = dh;
+    }
+    __syncthreads();  // every wave is done with the previous row's x
+#pragma unroll
+    for (int u = 0; u < kStage; ++u)
+      if (tid + u * kThreads < R) s_x[tid + u * kThreads] = stage[u];
+    __syncthreads();
+#else
     const int R = (int)C(L, kl);
     const int i0 = (int)(task >> 24) * kChunk;
     __syncthreads();
     if (tid < 64) {
       for (int b = tid; b < NHIGH; b += 64) {
         const bool anti = __builtin_popcount(h & HIM[b]) == 1;
-        s_hbase[b] = (anti && flip[HIK[b]]) ? rowbase(h ^ HIM[b]) : kNone;
+        s_hbase[0][b] = (anti && flip[HIK[b]]) ? rowbase(h ^ HIM[b]) : kNone;
       }
     } else if (tid < 64 + NH) {
-      s_cbase[tid - 64] = rowbase(h ^ (1u << (tid - 64)));
+      s_cbase[0][tid - 64] = rowbase(h ^ (1u << (tid - 64)));
     } else if (tid == 192) {
       s_B = rowbase(h);
       int c0 = 0, c1 = 0;
 #pragma unroll
       for (int b = 0; b < NHIGH; ++b)
         if (__builtin_popcount(h & HIM[b]) == 1) { if (HIK[b]) ++c1; else ++c0; }
-      s_cnt0 = c0;
-      s_cnt1 = c1;
+      s_cnt0[0] = c0;
+      s_cnt1[0] = c1;
       double dh = 0.0;
       if (DIAG_MODE == 1)
         for (int q = 0; q < NH; ++q) dh = ((h >> q) & 1u) ? dh + s_f[L + q] : dh - s_f[L + q];
-      s_dish = dh;
+      s_dish[0] = dh;
     }
     __syncthreads();
     if (tid == 0) {
       int n = 0;
       for (int b = 0; b < NHIGH; ++b)
-        if (s_hbase[b] != kNone) { s_hbase[n] = s_hbase[b]; s_hcoef[n] = jh_[HIK[b]]; ++n; }
-      s_nact = n;
+        if (s_hbase[0][b] != kNone) { s_hbase[0][n] = s_hbase[0][b]; s_hcoef[0][n] = jh_[HIK[b]]; ++n; }
+      s_nact[0] = n;
     }
     __syncthreads();
     const BT B = rfl(s_B);
-#if ROWSTAGE
-    const int Rrow = (task >> 24) ? 0 : R;  // padding items (chunk field 0xff) hold no row
-    for (int e = tid; e < Rrow; e += kThreads) s_x[e] = xv[B + (BT)e];
-    __syncthreads();
 #endif
-    const int nact = __builtin_amdgcn_readfirstlane(s_nact);
-    const int cnt0 = __builtin_amdgcn_readfirstlane(s_cnt0), cnt1 = __builtin_amdgcn_readfirstlane(s_cnt1);
+    const int nact = __builtin_amdgcn_readfirstlane(s_nact[pb]);
+    const int cnt0 = __builtin_amdgcn_readfirstlane(s_cnt0[pb]), cnt1 = __builtin_amdgcn_readfirstlane(s_cnt1[pb]);
     BT cb[NH + 1];
 #pragma unroll
-    for (int q = 0; q < NH; ++q) cb[q] = rfl(s_cbase[q]);
+    for (int q = 0; q < NH; ++q) cb[q] = rfl(s_cbase[pb][q]);
     const unsigned short* Uk = a.U + a.offU[kl];
 #pragma unroll 1
 #if ROWSTAGE
-    for (int k = 0; k < (RMAX + kThreads - 1) / kThreads; ++k) {
+    for (int k = 0; k < kStage; ++k) {
       const int i = tid + k * kThreads;
-      if (i >= Rrow) break;
+      if (i >= R) break;
 #else
     for (int k = 0; k < kPerLane; ++k) {
       const int i = i0 + tid + k * kThreads;
@@ -217,11 +299,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           v[u] = 0.0;
-          if (b0 + u < nact) v[u] = xv[rfl(s_hbase[b0 + u]) + (BT)i];
+          if (b0 + u < nact) v[u] = xv[rfl(s_hbase[pb][b0 + u]) + (BT)i];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          if (b0 + u < nact) acc += s_hcoef[b0 + u] * v[u];
+          if (b0 + u < nact) acc += s_hcoef[pb][b0 + u] * v[u];
       }
       const unsigned int l = Uk[i];
       const unsigned int v0 = l & 255u;
@@ -322,7 +404,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         double dis = 0.0;
 #pragma unroll
         for (int q = 0; q < L; ++q) dis = ((l >> q) & 1u) ? dis + s_f[q] : dis - s_f[q];
-        dis += s_dish;
+        dis += s_dish[pb];
         d = (dq0_ * (double)c0 + dq1_ * (double)c1) + 0.5 * dis;
       }
 #if ROWSTAGE
@@ -413,12 +495,13 @@ struct JitModule {
   hipModule_t mod = nullptr;
   hipFunction_t fn_dot = nullptr, fn_nodot = nullptr;
   bool ok = false;
-  bool rowstage = false;  // one 1024-thread block per row (rows_ord), else 256 threads per item
+  bool rowstage = false;  // one block per row (rows_ord), else 256 threads per item
+  int threads = 256;      // block size
   std::string log;
 };
 
-// The single-model kernel stages each row of x in LDS (one 1024-thread block
-// per row) unless LSQED_JIT_ROWSTAGE=0.  The low bonds then read LDS instead
+// The single-model kernel stages each row of x in LDS (persistent 1024-thread
+// blocks walking whole rows) unless LSQED_JIT_ROWSTAGE=0.  The low bonds then read LDS instead
 // of issuing global gathers: 37% fewer vector memory instructions on the
 // 4x8 J1-J2 sector, whose H.v was bound by address processing (TA busy 92%);
 // 43.7 -> 41.0 ms there, 779 -> 736 ms on 6x6, 0.185 -> 0.152 ms on 4x6,
@@ -426,6 +509,15 @@ struct JitModule {
 static bool jit_rowstage() {
   const char* e = getenv("LSQED_JIT_ROWSTAGE");
   return !(e && atoi(e) == 0);
+}
+
+// Threads of a row-staged block: 256 for rows of at most C(14, 7) entries
+// (four blocks per CU), 1024 above (one block per CU holds a C(16, 8) row);
+// LSQED_JIT_ROW_THREADS (256, 512 or 1024) overrides.
+static int jit_row_threads(int L) {
+  const char* e = getenv("LSQED_JIT_ROW_THREADS");
+  const int v = e ? atoi(e) : (L <= 14 ? 256 : 1024);
+  return (v == 256 || v == 512) ? v : 1024;
 }
 
 static std::mutex g_jit_mu;
@@ -448,10 +540,10 @@ static std::string dbl(double v) {
 }
 
 // Source of the kernel specialised to the bond classes and couplings of `h`
-// (row split with L = min(16, N) low bits, see hv_split.hip).  Fields stay a
+// (row split with L = split_low_bits(h) low bits, see hv_split.hip).  Fields stay a
 // run-time array, so disorder realisations share one compiled kernel.
 std::string jit_source(const lsqed_ham* h, bool batch) {
-  const int N = h->N, nup = h->nup, L = std::min(16, N), NH = N - L;
+  const int N = h->N, nup = h->nup, L = split_low_bits(h), NH = N - L;
   std::vector<unsigned> lowm, him;
   std::vector<int> lowk, lowt, crq, crp, crk, hik;
   for (int kind = 0; kind < 2; ++kind)
@@ -481,7 +573,8 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
     if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
   if (!batch && jit_rowstage())
-    o << "#define ROWSTAGE 1\nconstexpr int RMAX = " << binom_u64(L, L / 2) << ";\n";
+    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L) << "\nconstexpr int RMAX = "
+      << binom_u64(L, L / 2) << ";\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
@@ -542,6 +635,8 @@ static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch
   }
   m->ok = true;
   m->rowstage = src.find("#define ROWSTAGE 1") != std::string::npos;
+  const size_t rt = src.find("#define ROW_THREADS ");
+  m->threads = m->rowstage && rt != std::string::npos ? atoi(src.c_str() + rt + 20) : 256;
   return m;
 }
 
@@ -571,14 +666,21 @@ int jit_prepare(lsqed_ham* h) {
   return LSQED_OK;
 }
 
-// Blocks of one specialised H.v (one per work item, or one per row when the
-// kernel stages rows); also the number of x.y partial sums it writes.
+// Blocks of one specialised H.v (one per work item, or a persistent block per
+// CU when the kernel stages rows); also the number of x.y partial sums it
+// writes.
 int jit_grid(lsqed_ham* h) {
   if (jit_prepare(h) != LSQED_OK) return 0;
   SplitTables t;
   if (split_tables(h, t) != LSQED_OK) return 0;
-  const int n = ((JitModule*)h->jit)->rowstage ? t.nrows_ord : t.ntasks_ord;
-  return std::max(8, ((n + 7) / 8) * 8);
+  if (((JitModule*)h->jit)->rowstage) {
+    // one block per row: the dispatcher balances the uneven row lengths (a
+    // persistent grid of one block per CU dealing rows cyclically measured
+    // 7% slower on the 4x8 sector: a fixed share of rows per CU is unevenly
+    // long)
+    return t.nrows_ord;
+  }
+  return std::max(8, ((t.ntasks_ord + 7) / 8) * 8);
 }
 
 int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s) {
@@ -602,7 +704,7 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
   a.ntasks = m->rowstage ? t.nrows_ord : t.ntasks_ord;
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
   void* args[] = {&a};
-  LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, m->rowstage ? 1024 : 256,
+  LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, m->threads,
                                   1, 1, 0, s, args, nullptr));
   return LSQED_OK;
 }

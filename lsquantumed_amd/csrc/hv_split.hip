@@ -1,7 +1,7 @@
 // hv_split.hip -- row-split matrix-free H.v for N <= 32 (the fast path).
 //
 // The colex rank separates into a high and a low part.  Write a state as
-// s = (h << L) | l with L = min(16, N) low bits.  With kl = nup - popc(h),
+// s = (h << L) | l with L <= 16 low bits (split_low_bits).  With kl = nup - popc(h),
 //   r(s) = B(h) + RL(l),
 //   B(h)  = sum_{j-th set bit of h at p} C(L + p, kl + j)   (row base)
 //   RL(l) = sum_{j-th set bit of l at q} C(q, j)            (index inside the row)
@@ -46,8 +46,8 @@ struct SplitParams {
   int nlow, ncross, nhigh;
   int offU[17];
   uint32_t low[kMaxClass];    // mask (16 bits) | kind << 16 | type << 17 (0: byte 0, 1: byte 1, 2: split)
-  uint32_t cross[kMaxClass];  // q | p << 4 | kind << 8
-  uint32_t high[kMaxClass];   // mask on h (16 bits) | kind << 16
+  uint32_t cross[kMaxClass];  // q | p << 8 | kind << 16
+  uint32_t high[kMaxClass];   // mask on h (NH <= 24 bits) | kind << 31
   double jh[2], dq[2];
   int flip[2], dz_on[2], nbk[2];
   int diag_mode;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, 
   __shared__ double s_f[64];
   __shared__ uint32_t s_hbase[kMaxClass];
   __shared__ double s_hcoef[kMaxClass];
-  __shared__ uint32_t s_cbase[16];
+  __shared__ uint32_t s_cbase[24];  // NH = N - L <= 24 (N <= 32, L >= 8)
   __shared__ int s_nact;
   __shared__ uint32_t s_B;
   __shared__ int s_cnt[2];
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, 
     __syncthreads();  // previous item's LDS reads done
     if (tid < p.nhigh) {
       const uint32_t hd = p.high[tid];
-      const uint32_t m = hd & 0xffffu;
+      const uint32_t m = hd & 0xffffffu;
       const bool anti = __builtin_popcount(h & m) == 1;
       s_hbase[tid] = anti ? rowbase(h ^ m) : 0xffffffffu;
     } else if (tid >= 64 && tid < 64 + p.NH) {
@@ -139,8 +139,8 @@ __global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, 
       int c0 = 0, c1 = 0;
       for (int b = 0; b < p.nhigh; ++b) {
         const uint32_t hd = p.high[b];
-        if (__builtin_popcount(h & (hd & 0xffffu)) == 1) {
-          if ((hd >> 16) & 1) ++c1; else ++c0;
+        if (__builtin_popcount(h & (hd & 0xffffffu)) == 1) {
+          if (hd >> 31) ++c1; else ++c0;
         }
       }
       s_cnt[0] = c0;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, 
       int n = 0;
       for (int b = 0; b < p.nhigh; ++b) {
         const uint32_t base = s_hbase[b];
-        const int kind = (p.high[b] >> 16) & 1;
+        const int kind = (int)(p.high[b] >> 31);
         if (base != 0xffffffffu && p.flip[kind]) {
           s_hbase[n] = base;
           s_hcoef[n] = p.jh[kind];
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kSplitThreads) void hv_split_kernel(SplitParams p, 
       for (int b = 0; b < kMaxClass; ++b) {
         if (b < p.ncross) {
           const uint32_t cd = p.cross[b];
-          const int q = cd & 15, hp = (cd >> 4) & 15, kind = (cd >> 8) & 1;
+          const int q = cd & 255, hp = (cd >> 8) & 255, kind = (cd >> 16) & 1;
           const uint32_t bq = (l >> q) & 1u, bp = (h >> hp) & 1u;
           if (bq != bp) {
             if (kind) ++cc1; else ++cc0;
@@ -395,7 +395,7 @@ int split_plan_create(lsqed_ham* h) {
   if (h->split) return LSQED_OK;
   if (!split_plan_supported(h)) return fail(LSQED_ERR_ARG, "row-split tables need 2 <= N <= 40");
   const int N = h->N, nup = h->nup;
-  const int L = std::min(16, N), NH = N - L;
+  const int L = split_low_bits(h), NH = N - L;
   SplitPlan* sp = new SplitPlan;
   SplitParams& p = sp->p;
   std::memset(&p, 0, sizeof(p));
@@ -450,10 +450,10 @@ int split_plan_create(lsqed_ham* h) {
         p.low[p.nlow++] = ((1u << lo) | (1u << hi)) | ((uint32_t)kind << 16) | ((uint32_t)type << 17);
       } else if (lo >= L) {
         if (p.nhigh == kMaxClass) { sp->classes_ok = false; break; }
-        p.high[p.nhigh++] = ((1u << (lo - L)) | (1u << (hi - L))) | ((uint32_t)kind << 16);
+        p.high[p.nhigh++] = ((1u << (lo - L)) | (1u << (hi - L))) | ((uint32_t)kind << 31);
       } else {
         if (p.ncross == kMaxClass) { sp->classes_ok = false; break; }
-        p.cross[p.ncross++] = (uint32_t)lo | ((uint32_t)(hi - L) << 4) | ((uint32_t)kind << 8);
+        p.cross[p.ncross++] = (uint32_t)lo | ((uint32_t)(hi - L) << 8) | ((uint32_t)kind << 16);
       }
     }
   }
@@ -514,13 +514,30 @@ int split_tables(lsqed_ham* h, SplitTables& t) {
   return LSQED_OK;
 }
 
+// Low bits of the row split: max(14, N - 24) for N >= 24 (a work item holds
+// the high pattern in 24 bits), else min(16, N); LSQED_SPLIT_L
+// (8..16, keeping at most 24 high bits) overrides.  Rows of C(14, 7) = 3432
+// entries let four row-staged blocks share a CU (LDS), against one at L = 16
+// (C(16, 8) = 12870 entries, 103 KB): the specialised H.v measured 39.4 vs
+// 40.5 ms on the 4x8 J1-J2 sector, 717 vs 721 ms on 6x6, 0.140 vs 0.143 ms on
+// 4x6, although two more bonds cross the split; L = 13 and 12 were slower
+// (42.4, 47.3 ms on 4x8).
+int split_low_bits(const lsqed_ham* h) {
+  int L = h->N >= 24 ? std::max(14, h->N - 24) : std::min(16, h->N);
+  if (const char* e = std::getenv("LSQED_SPLIT_L")) {
+    const int v = std::atoi(e);
+    if (v >= 8 && v <= 16 && v <= h->N && h->N - v <= 24) L = v;
+  }
+  return L;
+}
+
 // The specialised kernel handles up to 64 high bonds (one prologue lane each).
 bool split_plan_supported(const lsqed_ham* h) { return h->N >= 2 && h->N <= 40; }
 
 bool split_jit_fits(const lsqed_ham* h) {
   if (!split_plan_supported(h)) return false;
   if ((int)h->bi[0].size() + (int)h->bi[1].size() > 2 * kMaxBonds) return false;
-  const int L = std::min(16, h->N);
+  const int L = split_low_bits(h);
   int nhigh = 0;
   for (int k = 0; k < 2; ++k)
     for (size_t b = 0; b < h->bi[k].size(); ++b)

@@ -137,6 +137,7 @@ struct SplitTables {
   const int* offU;
 };
 bool split_supported(const lsqed_ham* h);
+int split_low_bits(const lsqed_ham* h);
 bool split_plan_supported(const lsqed_ham* h);
 int split_tables(lsqed_ham* h, SplitTables& t);
 int split_plan_create(lsqed_ham* h);
