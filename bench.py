@@ -621,6 +621,7 @@ def main():
     torch.cuda.synchronize()
     hv_ms = ev0.elapsed_time(ev1) / args.steps
     hv_gbs = B / (hv_ms * 1e-3) / 1e9
+    traffic = measured_traffic(kernel_name, lx, ly)
 
     # --- Lanczos steps ------------------------------------------------------
     sp, beta = 0.0, 0.0
@@ -676,7 +677,10 @@ def main():
         "lanczos_steps_per_s": round(steps_per_s, 3),
         "lanczos_wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(hv_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_name, lx, ly),
+                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     # measured HBM bytes per launch over the same launch time (the algorithmic bytes
+                     # count every gather, also those served from cache)
+                     "traffic_gbs": round(traffic / (hv_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "measured_read_stream_gbs": stream["read_gbs"],
                      "measured_triad_gbs": max(stream["triad_gbs"], stream["torch_triad_gbs"])},
         "stream": stream,

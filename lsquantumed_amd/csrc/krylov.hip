@@ -143,12 +143,13 @@ __global__ __launch_bounds__(kThreads) void k_update(const double* __restrict__ 
 //   MODE 0: part[j] += V_j . w                          (projection)
 //   MODE 1: w <- w - V h;  part[j] += V_j . w; part[m] += ||w||^2
 //           (the first update fused with the second projection of CGS2)
-//   MODE 2: w <- w - V h;  part[0] += ||w||^2           (final update)
+//   MODE 2: w <- (w - V h) * sc;  part[0] += ||w||^2     (final update; sc
+//           normalises the new basis vector in the same pass, 1 keeps it)
 // part is column-major over blocks: part[j * gridDim.x + block].
 template <int MAXM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_cgs_pass(const double* __restrict__ V, int64_t ld, int m,
                                                        const double* __restrict__ hcoef, double* __restrict__ w,
-                                                       int64_t n, double* __restrict__ part) {
+                                                       int64_t n, double* __restrict__ part, double sc) {
   __shared__ double hs[MAXM];
   __shared__ double red[kThreads / 64][MAXM + 1];
   if (MODE != 0)
@@ -169,6 +170,7 @@ __global__ __launch_bounds__(kThreads) void k_cgs_pass(const double* __restrict_
 #pragma unroll
       for (int j = 0; j < MAXM; ++j) s += vin[j] * hs[j];
       wr -= s;
+      if (MODE == 2) wr *= sc;
       w[r] = wr;
       acc[MAXM] += wr * wr;
     }
@@ -459,53 +461,70 @@ int scale(Ctx& c, double* x, double a) {
 // next basis vector -- which is how degenerate copies of an eigenvalue enter
 // the Krylov space.
 template <int MAXM, int MODE>
-void launch_cgs_pass(Ctx& c, int G, int m, const double* hdev, double* w) {
+void launch_cgs_pass(Ctx& c, int G, int m, const double* hdev, double* w, double sc) {
   hipLaunchKernelGGL((k_cgs_pass<MAXM, MODE>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, m, hdev, w, c.D,
-                     c.e->part);
+                     c.e->part, sc);
 }
 
 // CGS2 in three passes over V for m <= 32 (projection; first update fused
 // with the second projection; second update), each with all column loads of
-// a row in flight.
+// a row in flight.  With `normalise`, the second update also divides by the
+// norm it leaves, known before the pass: w2 = w1 - V h2 with h2 = V^T w1 and
+// V orthonormal gives ||w2||^2 = ||w1||^2 - ||h2||^2, without cancellation
+// since the refinement coefficients h2 are small next to w1 unless w is
+// dependent on V (then the pass does not scale).  *scaled reports which;
+// nrm is the divisor, so that w_true = nrm * w exactly.
 int cgs2_fused(Ctx& c, int m, double* w, std::vector<double>& h1, std::vector<double>& h2, double& nrm1,
-               double& nrm) {
+               double& nrm, bool normalise = false, bool* scaled = nullptr) {
   const int G = red_blocks(c.D);
   double* hdev = c.e->dsm + kMaxNcv * kMaxNcv;  // coefficient staging (2 kMaxNcv doubles)
   std::vector<double> out(m + 1);
   int rc;
-  auto pass = [&](int mode, const std::vector<double>* hc) -> int {
+  auto pass = [&](int mode, const std::vector<double>* hc, double sc) -> int {
     if (hc) LSQED_HIP(hipMemcpyAsync(hdev, hc->data(), m * sizeof(double), hipMemcpyHostToDevice, c.s));
     if (m <= 16) {
-      if (mode == 0) launch_cgs_pass<16, 0>(c, G, m, hdev, w);
-      if (mode == 1) launch_cgs_pass<16, 1>(c, G, m, hdev, w);
-      if (mode == 2) launch_cgs_pass<16, 2>(c, G, m, hdev, w);
+      if (mode == 0) launch_cgs_pass<16, 0>(c, G, m, hdev, w, sc);
+      if (mode == 1) launch_cgs_pass<16, 1>(c, G, m, hdev, w, sc);
+      if (mode == 2) launch_cgs_pass<16, 2>(c, G, m, hdev, w, sc);
     } else if (m <= 24) {
-      if (mode == 0) launch_cgs_pass<24, 0>(c, G, m, hdev, w);
-      if (mode == 1) launch_cgs_pass<24, 1>(c, G, m, hdev, w);
-      if (mode == 2) launch_cgs_pass<24, 2>(c, G, m, hdev, w);
+      if (mode == 0) launch_cgs_pass<24, 0>(c, G, m, hdev, w, sc);
+      if (mode == 1) launch_cgs_pass<24, 1>(c, G, m, hdev, w, sc);
+      if (mode == 2) launch_cgs_pass<24, 2>(c, G, m, hdev, w, sc);
     } else {
-      if (mode == 0) launch_cgs_pass<32, 0>(c, G, m, hdev, w);
-      if (mode == 1) launch_cgs_pass<32, 1>(c, G, m, hdev, w);
-      if (mode == 2) launch_cgs_pass<32, 2>(c, G, m, hdev, w);
+      if (mode == 0) launch_cgs_pass<32, 0>(c, G, m, hdev, w, sc);
+      if (mode == 1) launch_cgs_pass<32, 1>(c, G, m, hdev, w, sc);
+      if (mode == 2) launch_cgs_pass<32, 2>(c, G, m, hdev, w, sc);
     }
     LSQED_HIP(hipGetLastError());
     const int ncol = mode == 0 ? m : (mode == 1 ? m + 1 : 1);
     return finish_reduce(c.e->h, c.e->part, G, ncol, c.e->dsm, out.data(), c.s);
   };
-  if ((rc = pass(0, nullptr))) return rc;
+  if ((rc = pass(0, nullptr, 1.0))) return rc;
   h1.assign(out.begin(), out.begin() + m);
-  if ((rc = pass(1, &h1))) return rc;
+  if ((rc = pass(1, &h1, 1.0))) return rc;
   h2.assign(out.begin(), out.begin() + m);
   nrm1 = std::sqrt(out[m]);
-  if ((rc = pass(2, &h2))) return rc;
-  nrm = std::sqrt(out[0]);
+  double beta = 0.0;
+  if (normalise) {
+    double hh = 0.0;
+    for (int j = 0; j < m; ++j) hh += h2[j] * h2[j];
+    const double b2 = out[m] - hh;
+    if (b2 > 0.0 && std::sqrt(b2) >= 0.7071067811865476 * nrm1) beta = std::sqrt(b2);
+  }
+  if ((rc = pass(2, &h2, beta > 0.0 ? 1.0 / beta : 1.0))) return rc;
+  if (scaled) *scaled = beta > 0.0;
+  nrm = beta > 0.0 ? beta : std::sqrt(out[0]);
   return LSQED_OK;
 }
 
-int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* lindep = nullptr) {
+// With `scaled` non-null, w comes back normalised when *scaled is set (see
+// cgs2_fused); otherwise w is left unscaled with norm nrm.
+int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* lindep = nullptr,
+         bool* scaled = nullptr) {
   std::vector<double> h1, h2;
   int rc;
   if (lindep) *lindep = false;
+  if (scaled) *scaled = false;
   if (m == 0) {
     rc = norm2(c, w, nrm);
     if (lindep) *lindep = !(nrm > 0.0);
@@ -513,7 +532,7 @@ int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* li
   }
   double nrm1 = 0.0;
   if (m <= 32) {
-    if ((rc = cgs2_fused(c, m, w, h1, h2, nrm1, nrm))) return rc;
+    if ((rc = cgs2_fused(c, m, w, h1, h2, nrm1, nrm, scaled != nullptr, scaled))) return rc;
   } else {
     if ((rc = multidot(c, m, w, h1))) return rc;
     if ((rc = update(c, m, w, h1, nrm1))) return rc;
@@ -648,8 +667,8 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
       if ((rc = launch_hv(e->h, col(j), col(j + 1), nullptr, c.s))) return rc;
       std::vector<double> hcoef;
       double b;
-      bool lindep = false;
-      if ((rc = cgs2(c, j + 1, col(j + 1), hcoef, b, &lindep))) return rc;
+      bool lindep = false, scaled = false;
+      if ((rc = cgs2(c, j + 1, col(j + 1), hcoef, b, &lindep, &scaled))) return rc;
       T[(size_t)j * m + j] = hcoef[j];
       if (j == k) {
         for (int i = 0; i < k; ++i) T[(size_t)i * m + k] = T[(size_t)k * m + i] = arrow[i];
@@ -662,7 +681,7 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
         breakdown = true;
         break;
       }
-      if ((rc = scale(c, col(j + 1), 1.0 / b))) return rc;
+      if (!scaled && (rc = scale(c, col(j + 1), 1.0 / b))) return rc;
     }
     ++e->its;
     // projected problem
