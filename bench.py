@@ -503,21 +503,29 @@ def lanczos_6x6(maxit: int = 600, tol: float = 1e-9):
             "device_gb": round(3 * D * 8 / 1e9, 1)}
 
 
-def hv_only(lx: int, ly: int, reps: int, warmup: int):
-    """H.v alone on the periodic lx x ly square lattice (J1=D1=1, J2=D2=0.5,
-    Sz=0) with two device vectors, timed with HIP events over `reps` launches
-    on the launch stream; x.(Hx) is reported as a checksum.  The row split
-    and block size follow LSQED_SPLIT_L / LSQED_JIT_ROW_THREADS when set."""
+def hv_only(lx: int, ly: int, reps: int, warmup: int, chain: int = 0, hv_kernel: int = 0):
+    """H.v alone with two device vectors, timed with HIP events over `reps`
+    launches on the launch stream; x.(Hx) is reported as a checksum.  The
+    periodic lx x ly square lattice (J1=D1=1, J2=D2=0.5), or with `chain` > 0
+    the periodic chain of that many spins (J1=D1=1, J2=D2=0.2, the disorder
+    workload's couplings, no fields); Sz=0.  The row split and block size
+    follow LSQED_SPLIT_L / LSQED_JIT_ROW_THREADS when set."""
     import torch
 
     import lsquantumed_amd as Q
 
-    N = lx * ly
+    N = chain if chain > 0 else lx * ly
     env = Q.Environment(N)
-    lat = Q.square2D(env, lx, ly)
-    h = Q.Hamiltonian(env, Q.Basis(env, 0), lat, 1.0, 1.0, 0.5, 0.5)
+    if chain > 0:
+        lat = Q.chain1D(env)
+        h = Q.Hamiltonian(env, Q.Basis(env, 0), lat, 1.0, 1.0, 0.2, 0.2)
+    else:
+        lat = Q.square2D(env, lx, ly)
+        h = Q.Hamiltonian(env, Q.Basis(env, 0), lat, 1.0, 1.0, 0.5, 0.5)
     h.build_diag(env)
     h.build_off_diag(env)
+    if hv_kernel:
+        h.set_hv_kernel(hv_kernel)
     D = h.hamilt_size()
     B, _, _ = algorithmic_bytes(N, N // 2, len(lat.nn_bonds[0]) + len(lat.nnn_bonds[0]))
     g = torch.Generator(device="cuda").manual_seed(3)
@@ -536,7 +544,9 @@ def hv_only(lx: int, ly: int, reps: int, warmup: int):
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / reps
     kernel = {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()]
-    return {"workload": f"H.v, periodic square {lx}x{ly} (N={N}), J1=D1=1, J2=D2=0.5, Sz=0, D={D}",
+    name = f"periodic chain N={N}, J1=D1=1, J2=D2=0.2" if chain > 0 else \
+        f"periodic square {lx}x{ly} (N={N}), J1=D1=1, J2=D2=0.5"
+    return {"workload": f"H.v, {name}, Sz=0, D={D}",
             "hv_kernel": kernel, "split_low_bits": os.environ.get("LSQED_SPLIT_L", "default"),
             "row_threads": os.environ.get("LSQED_JIT_ROW_THREADS", "default"), "hv_ms": round(ms, 4),
             "algorithmic_bytes": B, "gbs": round(B / ms / 1e6, 1), "roofline_frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
@@ -561,6 +571,7 @@ def main():
                     help="only Krylov-Schur nev=4 ncv=20 on the 4x8 sector (101 GB basis; about a minute)")
     ap.add_argument("--hv-only", action="store_true",
                     help="only H.v on the --lx x --ly lattice with two vectors (6x6: 145 GB); one JSON object")
+    ap.add_argument("--chain", type=int, default=0, help="with --hv-only: a periodic chain of this many spins")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Sz-sector path here is built and measured on a single "
@@ -582,7 +593,7 @@ def main():
         print(json.dumps({"krylov_schur_4x8": krylov_schur_4x8()}))
         return
     if args.hv_only:
-        print(json.dumps({"hv": hv_only(args.lx, args.ly, args.steps, args.warmup)}))
+        print(json.dumps({"hv": hv_only(args.lx, args.ly, args.steps, args.warmup, args.chain, args.hv_kernel)}))
         return
     lx, ly = args.lx, args.ly
     N = lx * ly

@@ -537,7 +537,7 @@ int setup_groups(lsqed_batch* b) {
   std::vector<int> pl(np, 0);
   for (int k = 0; k < np; ++k) {
     lsqed_ham* h = b->hams[k];
-    if (!split_supported(h) || !split_jit_fits(h)) return LSQED_OK;
+    if (!split_supported(h) || !split_jit_fits(h, true)) return LSQED_OK;
     void* mod = nullptr;
     std::string key;
     if (batch_jit_prepare(h, &mod, &key) != LSQED_OK) return LSQED_OK;
@@ -556,7 +556,7 @@ int setup_groups(lsqed_batch* b) {
   int part = 0, gofs = 0;
   std::vector<int> gidx;
   for (Group& g : b->groups) {
-    const int rc = split_task_count(g.h0, &g.ntasks);
+    const int rc = batch_task_count(g.module, g.h0, &g.ntasks);
     if (rc) return rc;
     g.part0 = part;
     g.gofs = gofs;

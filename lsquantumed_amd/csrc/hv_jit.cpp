@@ -138,6 +138,9 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   const long long lb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
   const long long ntot = (long long)a.nprob * a.ntasks;
   int loaded = -1;
+#if ROWSTAGE
+  int pb = 0;  // table buffer of the block's current row (alternates per row)
+#endif
   for (long long t = lb; t < ntot; t += G) {
     const int pl = (int)(t / a.ntasks);
     const int st = (int)(t - (long long)pl * a.ntasks);
@@ -151,9 +154,14 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       __syncthreads();
       for (int i = tid; i < N; i += kThreads) s_f[i] = a.pfields[32 * prob + i];
       loaded = prob;
+#if ROWSTAGE
+      __syncthreads();  // the row prologue reads the fields before its own barrier
+#endif
     }
     double dotacc = 0.0;
+#if !ROWSTAGE
     constexpr int pb = 0;
+#endif
     const unsigned int task = a.tasks[st];
 #elif ROWSTAGE
   // The row list holds eight equal segments, one per XCD; block b runs on
@@ -187,17 +195,12 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #if ROWSTAGE
     const int R = (task >> 24) ? 0 : (int)C(L, kl);  // padding items (chunk field 0xff) hold no row
     const int i0 = 0;
-    // every thread computes the row base and issues its staging loads now;
     // wave 0 (high bonds, compacted with a ballot) and wave 1 (cross rows,
-    // field sum) fill this row's table buffer meanwhile
+    // field sum) fill this row's table buffer while the previous row's waves
+    // finish; then every thread issues its staging loads, which are in
+    // flight across the barrier (issued after the tables, so that the staged
+    // values and the table work are not live together)
     const BT B = rfl(rowbase(h));
-    double stage[kStage];
-#pragma unroll
-    for (int u = 0; u < kStage; ++u) {
-      const int e = tid + u * kThreads < R ? tid + u * kThreads : R - 1;
-      stage[u] = 0.0;
-      if (u * kThreads < R) stage[u] = xv[B + (BT)e];
-    }
     if (tid < 64) {
       int nact = 0, c0 = 0, c1 = 0;
 #pragma unroll
@@ -235,6 +238,13 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       if (DIAG_MODE == 1)
         for (int q = 0; q < NH; ++q) dh = ((h >> q) & 1u) ? dh + s_f[L + q] : dh - s_f[L + q];
       s_dish[pb] = dh;
+    }
+    double stage[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = tid + u * kThreads < R ? tid + u * kThreads : R - 1;
+      stage[u] = 0.0;
+      if (u * kThreads < R) stage[u] = xv[B + (BT)e];
     }
     __syncthreads();  // every wave is done with the previous row's x
 #pragma unroll
@@ -430,6 +440,9 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         a.dot[t] = tot;
       }
     }
+#if ROWSTAGE
+    pb ^= 1;
+#endif
 #endif
   }
 #if !BATCH
@@ -540,10 +553,10 @@ static std::string dbl(double v) {
 }
 
 // Source of the kernel specialised to the bond classes and couplings of `h`
-// (row split with L = split_low_bits(h) low bits, see hv_split.hip).  Fields stay a
+// (row split with L = split_low_bits(h, batch) low bits, see hv_split.hip).  Fields stay a
 // run-time array, so disorder realisations share one compiled kernel.
 std::string jit_source(const lsqed_ham* h, bool batch) {
-  const int N = h->N, nup = h->nup, L = split_low_bits(h), NH = N - L;
+  const int N = h->N, nup = h->nup, L = split_low_bits(h, batch), NH = N - L;
   std::vector<unsigned> lowm, him;
   std::vector<int> lowk, lowt, crq, crp, crk, hik;
   for (int kind = 0; kind < 2; ++kind)
@@ -572,9 +585,9 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
     if (atoi(w) >= 1 && atoi(w) <= 8) o << "#define WAVES_PER_EU " << atoi(w) << "\n";
   if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
     if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
-  if (!batch && jit_rowstage())
-    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L) << "\nconstexpr int RMAX = "
-      << binom_u64(L, L / 2) << ";\n";
+  if (jit_rowstage() && (!batch || L <= 14))
+    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << (batch ? 256 : jit_row_threads(L))
+      << "\nconstexpr int RMAX = " << binom_u64(L, L / 2) << ";\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
@@ -674,11 +687,20 @@ int jit_grid(lsqed_ham* h) {
   SplitTables t;
   if (split_tables(h, t) != LSQED_OK) return 0;
   if (((JitModule*)h->jit)->rowstage) {
-    // one block per row: the dispatcher balances the uneven row lengths (a
-    // persistent grid of one block per CU dealing rows cyclically measured
-    // 7% slower on the 4x8 sector: a fixed share of rows per CU is unevenly
-    // long)
-    return t.nrows_ord;
+    // two rows per block when there are at least 2^15 rows, else one
+    // (LSQED_JIT_ROWS_PER_BLOCK overrides): a block's once-per-block work --
+    // its tables, and the stores of the 12 VGPRs the kernel spills at four
+    // waves per SIMD (3.2 GB of scratch writes per 4x8 product at one row
+    // per block) -- is shared by two rows, while the dispatcher still
+    // balances the uneven row lengths.  Rows per block on 4x8 / 6x6: 1: 39.5
+    // / 701.7 ms, 2: 39.1 / 688.8 ms, 4: 38.6 / 714.1 ms, 8: 40.8 ms on 4x8;
+    // on 4x6 (2^10 rows) 2 was 43% slower than 1.  A persistent grid (one
+    // block per CU dealing rows cyclically) measured 7% slower: a fixed share
+    // of rows per CU is unevenly long.
+    const char* e = getenv("LSQED_JIT_ROWS_PER_BLOCK");
+    const int k = e ? std::max(1, std::min(64, atoi(e))) : (t.nrows_ord >= 32768 ? 2 : 1);
+    const int per_xcd = t.nrows_ord / 8;
+    return 8 * std::max(1, (per_xcd + k - 1) / k);
   }
   return std::max(8, ((t.ntasks_ord + 7) / 8) * 8);
 }
@@ -731,17 +753,19 @@ int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key) {
   return LSQED_OK;
 }
 
-int split_task_count(lsqed_ham* h, int* ntasks) {
+// Tasks per problem of a batch module: rows when it stages rows, else
+// 2048-entry work items.
+int batch_task_count(void* module, lsqed_ham* h, int* ntasks) {
   SplitTables t;
-  const int rc = split_tables(h, t);
+  const int rc = split_tables(h, t, true);
   if (rc) return rc;
-  *ntasks = t.ntasks;
+  *ntasks = ((JitModule*)module)->rowstage ? t.nrows : t.ntasks;
   return LSQED_OK;
 }
 
 int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, int grid, hipStream_t s) {
   SplitTables t;
-  int rc = split_tables(h0, t);
+  int rc = split_tables(h0, t, true);
   if (rc) return rc;
   JitModule* m = (JitModule*)module;
   BatchJitArgs a;
@@ -754,8 +778,8 @@ int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, in
   a.base.RL1 = t.RL1;
   a.base.binom = t.binom;
   a.base.binom64 = t.binom64;
-  a.base.tasks = t.tasks;
-  a.base.ntasks = t.ntasks;
+  a.base.tasks = m->rowstage ? t.rows : t.tasks;
+  a.base.ntasks = m->rowstage ? t.nrows : t.ntasks;
   for (int k = 0; k < 17; ++k) a.base.offU[k] = t.offU[k];
   a.poff = L.poff;
   a.pcoup = L.pcoup;
@@ -764,8 +788,8 @@ int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, in
   a.gidx = L.gidx;
   a.nprob = L.nprob;
   void* args[] = {&a};
-  LSQED_HIP(hipModuleLaunchKernel(L.dot ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, args,
-                                  nullptr));
+  LSQED_HIP(hipModuleLaunchKernel(L.dot ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, m->threads, 1, 1, 0, s,
+                                  args, nullptr));
   return LSQED_OK;
 }
 

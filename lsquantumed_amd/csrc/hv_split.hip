@@ -67,6 +67,8 @@ struct SplitPlan {
   int ntasks_ord = 0;               // a multiple of 8 (padding items process no entries)
   uint32_t* d_rows_ord = nullptr;   // whole rows, same order and padding (row-staged kernel)
   int nrows_ord = 0;
+  uint32_t* d_rows = nullptr;       // whole rows in h order, no padding (row-staged batched kernel)
+  int nrows = 0;
   size_t binom_n = 0;
 };
 
@@ -276,8 +278,12 @@ bool split_supported(const lsqed_ham* h) {
   return true;
 }
 
-void split_plan_destroy(lsqed_ham* h) {
-  SplitPlan* sp = (SplitPlan*)h->split;
+// The single-model kernels and the batched kernel use different row splits
+// (split_low_bits), so a model holds up to two plans.
+static void*& plan_slot(lsqed_ham* h, bool batch) { return batch ? h->split_b : h->split; }
+
+static void destroy_plan(void*& slot) {
+  SplitPlan* sp = (SplitPlan*)slot;
   if (!sp) return;
   (void)hipFree(sp->d_U);
   (void)hipFree(sp->d_RL0);
@@ -287,8 +293,14 @@ void split_plan_destroy(lsqed_ham* h) {
   (void)hipFree(sp->d_tasks);
   (void)hipFree(sp->d_tasks_ord);
   (void)hipFree(sp->d_rows_ord);
+  (void)hipFree(sp->d_rows);
   delete sp;
-  h->split = nullptr;
+  slot = nullptr;
+}
+
+void split_plan_destroy(lsqed_ham* h) {
+  destroy_plan(h->split);
+  destroy_plan(h->split_b);
 }
 
 // Order of the work items of one H.v.  A high or cross bond sends the
@@ -391,11 +403,12 @@ static void order_items(const lsqed_ham* h, int L, int NH, int ko, std::vector<u
   }
 }
 
-int split_plan_create(lsqed_ham* h) {
-  if (h->split) return LSQED_OK;
+int split_plan_create(lsqed_ham* h, bool batch) {
+  void*& slot = plan_slot(h, batch);
+  if (slot) return LSQED_OK;
   if (!split_plan_supported(h)) return fail(LSQED_ERR_ARG, "row-split tables need 2 <= N <= 40");
   const int N = h->N, nup = h->nup;
-  const int L = split_low_bits(h), NH = N - L;
+  const int L = split_low_bits(h, batch), NH = N - L;
   SplitPlan* sp = new SplitPlan;
   SplitParams& p = sp->p;
   std::memset(&p, 0, sizeof(p));
@@ -458,14 +471,16 @@ int split_plan_create(lsqed_ham* h) {
     }
   }
   // work items: (h, chunk) for every row with a valid low class
-  std::vector<uint32_t> tasks;
+  std::vector<uint32_t> tasks, rows;
   for (uint32_t hh = 0; hh < (1u << NH); ++hh) {
     const int kl = nup - count_bits(hh);
     if (kl < 0 || kl > L) continue;
     const int64_t R = (int64_t)binom_u64(L, kl);
     for (int64_t c = 0; c * kSplitChunk < R; ++c) tasks.push_back(hh | ((uint32_t)c << 24));
+    rows.push_back(hh);
   }
   sp->ntasks = (int)tasks.size();
+  sp->nrows = (int)rows.size();
   const int ko = outer_site_count(h, NH);
   std::vector<uint32_t> tasks_ord, rows_ord;
   order_items(h, L, NH, ko, tasks_ord, rows_ord);
@@ -481,9 +496,10 @@ int split_plan_create(lsqed_ham* h) {
       !up((void**)&sp->d_binom64, bin64.data(), bin64.size() * 8) ||
       !up((void**)&sp->d_tasks, tasks.data(), tasks.size() * 4) ||
       !up((void**)&sp->d_tasks_ord, tasks_ord.data(), tasks_ord.size() * 4) ||
-      !up((void**)&sp->d_rows_ord, rows_ord.data(), rows_ord.size() * 4)) {
-    h->split = sp;
-    split_plan_destroy(h);
+      !up((void**)&sp->d_rows_ord, rows_ord.data(), rows_ord.size() * 4) ||
+      !up((void**)&sp->d_rows, rows.data(), rows.size() * 4)) {
+    slot = sp;
+    destroy_plan(slot);
     return fail(LSQED_ERR_HIP, "row-split table upload failed");
   }
   p.U = sp->d_U;
@@ -491,14 +507,14 @@ int split_plan_create(lsqed_ham* h) {
   p.RL1 = sp->d_RL1;
   p.binom = sp->d_binom;
   p.tasks = sp->d_tasks_ord;
-  h->split = sp;
+  slot = sp;
   return LSQED_OK;
 }
 
-int split_tables(lsqed_ham* h, SplitTables& t) {
-  int rc = split_plan_create(h);
+int split_tables(lsqed_ham* h, SplitTables& t, bool batch) {
+  int rc = split_plan_create(h, batch);
   if (rc) return rc;
-  SplitPlan* sp = (SplitPlan*)h->split;
+  SplitPlan* sp = (SplitPlan*)plan_slot(h, batch);
   t.U = sp->d_U;
   t.RL0 = sp->d_RL0;
   t.RL1 = sp->d_RL1;
@@ -510,23 +526,41 @@ int split_tables(lsqed_ham* h, SplitTables& t) {
   t.ntasks_ord = sp->ntasks_ord;
   t.rows_ord = sp->d_rows_ord;
   t.nrows_ord = sp->nrows_ord;
+  t.rows = sp->d_rows;
+  t.nrows = sp->nrows;
   t.offU = sp->p.offU;
   return LSQED_OK;
 }
 
-// Low bits of the row split: max(14, N - 24) for N >= 24 (a work item holds
-// the high pattern in 24 bits), else min(16, N); LSQED_SPLIT_L
-// (8..16, keeping at most 24 high bits) overrides.  Rows of C(14, 7) = 3432
-// entries let four row-staged blocks share a CU (LDS), against one at L = 16
-// (C(16, 8) = 12870 entries, 103 KB): the specialised H.v measured 39.4 vs
-// 40.5 ms on the 4x8 J1-J2 sector, 717 vs 721 ms on 6x6, 0.140 vs 0.143 ms on
-// 4x6, although two more bonds cross the split; L = 13 and 12 were slower
-// (42.4, 47.3 ms on 4x8).
-int split_low_bits(const lsqed_ham* h) {
-  int L = h->N >= 24 ? std::max(14, h->N - 24) : std::min(16, h->N);
+// Low bits of the row split.  The batched kernel takes L = min(14, N) (its
+// rows are staged in LDS like the single-model kernel's; many problems give
+// the parallelism; LSQED_BATCH_SPLIT_L, 8..16, overrides -- above 14 it uses
+// 2048-entry work items without staging).  The single-model kernels (one row-staged block per
+// row) take L = clamp(N - 12, 10, 14) for N >= 20 and min(N, 10) below, and
+// at least N - 24 (a work item holds the high pattern in 24 bits), so that
+// there are about 2^12 rows or more to spread over the CUs while a row stays
+// within C(14, 7) = 3432 entries: four row-staged 256-thread blocks then share
+// a CU (LDS), against one at L = 16 (C(16, 8) = 12870 entries, 103 KB).
+// Measured (specialised H.v): 4x8 J1-J2 39.4 ms at L = 14 against 40.5 at 16,
+// 42.4 at 13, 47.3 at 12; 6x6 717 ms at 14 against 721 at 16 and 772 at 13;
+// periodic chains (J2 = 0.2) at L = 10 / 10 / 10 / 12 for N = 18 / 20 / 22 /
+// 24: 9.7 / 11.6 / 22.3 / 61.9 us against 49 / 94 / 105 / 100 us at
+// L = 16 / 16 / 16 / 14.  LSQED_SPLIT_L (8..16, keeping at most 24 high bits)
+// overrides the single-model choice.
+int split_low_bits(const lsqed_ham* h, bool batch) {
+  const int N = h->N;
+  if (batch) {
+    int L = std::min(14, N);
+    if (const char* e = std::getenv("LSQED_BATCH_SPLIT_L")) {
+      const int v = std::atoi(e);
+      if (v >= 8 && v <= 16 && v <= N && N - v <= 24) L = v;
+    }
+    return L;
+  }
+  int L = N >= 20 ? std::max(std::max(10, std::min(14, N - 12)), N - 24) : std::min(N, 10);
   if (const char* e = std::getenv("LSQED_SPLIT_L")) {
     const int v = std::atoi(e);
-    if (v >= 8 && v <= 16 && v <= h->N && h->N - v <= 24) L = v;
+    if (v >= 8 && v <= 16 && v <= N && N - v <= 24) L = v;
   }
   return L;
 }
@@ -534,10 +568,10 @@ int split_low_bits(const lsqed_ham* h) {
 // The specialised kernel handles up to 64 high bonds (one prologue lane each).
 bool split_plan_supported(const lsqed_ham* h) { return h->N >= 2 && h->N <= 40; }
 
-bool split_jit_fits(const lsqed_ham* h) {
+bool split_jit_fits(const lsqed_ham* h, bool batch) {
   if (!split_plan_supported(h)) return false;
   if ((int)h->bi[0].size() + (int)h->bi[1].size() > 2 * kMaxBonds) return false;
-  const int L = split_low_bits(h);
+  const int L = split_low_bits(h, batch);
   int nhigh = 0;
   for (int k = 0; k < 2; ++k)
     for (size_t b = 0; b < h->bi[k].size(); ++b)

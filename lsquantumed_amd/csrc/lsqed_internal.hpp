@@ -76,7 +76,8 @@ struct lsqed_ham {
   size_t scratch_doubles = 0;
   int device = 0;
   int hv_variant = 0;          // 0 auto, 1 generic chunk-table, 2 row split, 3 row split specialised (hipRTC)
-  void* split = nullptr;       // row-split plan (hv_split.hip), built on first use
+  void* split = nullptr;       // row-split plan of the single-model kernels (hv_split.hip), built on first use
+  void* split_b = nullptr;     // row-split plan of the batched kernel (its own row split)
   void* jit = nullptr;         // specialised row-split module (hv_jit.cpp), cached per bond structure
   bool jit_failed = false;
 };
@@ -134,17 +135,19 @@ struct SplitTables {
   int ntasks_ord;
   const uint32_t* rows_ord;  // whole rows in the same order (one block per row)
   int nrows_ord;
+  const uint32_t* rows;      // whole rows in h order, no padding
+  int nrows;
   const int* offU;
 };
 bool split_supported(const lsqed_ham* h);
-int split_low_bits(const lsqed_ham* h);
+int split_low_bits(const lsqed_ham* h, bool batch = false);
 bool split_plan_supported(const lsqed_ham* h);
-int split_tables(lsqed_ham* h, SplitTables& t);
-int split_plan_create(lsqed_ham* h);
+int split_tables(lsqed_ham* h, SplitTables& t, bool batch = false);
+int split_plan_create(lsqed_ham* h, bool batch = false);
 void split_plan_destroy(lsqed_ham* h);
 int split_grid(const lsqed_ham* h);
 int launch_hv_split(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
-bool split_jit_fits(const lsqed_ham* h);
+bool split_jit_fits(const lsqed_ham* h, bool batch = false);
 
 // Specialised row-split kernel (hv_jit.cpp).
 bool jit_enabled();
@@ -166,7 +169,7 @@ struct BatchJitLaunch {
   int nprob;              // problems in the group
 };
 int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key);
-int split_task_count(lsqed_ham* h, int* ntasks);
+int batch_task_count(void* module, lsqed_ham* h, int* ntasks);
 int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, int grid, hipStream_t s);
 
 // Reductions (krylov.hip)
