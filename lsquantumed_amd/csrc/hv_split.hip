@@ -535,18 +535,21 @@ int split_tables(lsqed_ham* h, SplitTables& t, bool batch) {
 // Low bits of the row split.  The batched kernel takes L = min(14, N) (its
 // rows are staged in LDS like the single-model kernel's; many problems give
 // the parallelism; LSQED_BATCH_SPLIT_L, 8..16, overrides -- above 14 it uses
-// 2048-entry work items without staging).  The single-model kernels (one row-staged block per
-// row) take L = clamp(N - 12, 10, 14) for N >= 20 and min(N, 10) below, and
-// at least N - 24 (a work item holds the high pattern in 24 bits), so that
-// there are about 2^12 rows or more to spread over the CUs while a row stays
-// within C(14, 7) = 3432 entries: four row-staged 256-thread blocks then share
-// a CU (LDS), against one at L = 16 (C(16, 8) = 12870 entries, 103 KB).
-// Measured (specialised H.v): 4x8 J1-J2 39.4 ms at L = 14 against 40.5 at 16,
-// 42.4 at 13, 47.3 at 12; 6x6 717 ms at 14 against 721 at 16 and 772 at 13;
-// periodic chains (J2 = 0.2) at L = 10 / 10 / 10 / 12 for N = 18 / 20 / 22 /
-// 24: 9.7 / 11.6 / 22.3 / 61.9 us against 49 / 94 / 105 / 100 us at
-// L = 16 / 16 / 16 / 14.  LSQED_SPLIT_L (8..16, keeping at most 24 high bits)
-// overrides the single-model choice.
+// 2048-entry work items without staging).  The single-model kernels (one
+// row-staged block per row) take, for N >= 20, L = clamp(N - 10, 10, 14) on
+// lattices with more than 2.5 flippable bonds per site and clamp(N - 11, 10,
+// 13) on sparser ones (a further low site moves only ~2 bonds into LDS there
+// while halving the rows), min(N, 10) below N = 20, and at least N - 24 (a
+// work item holds the high pattern in 24 bits): about 2^11-2^12 rows or more
+// to spread over the CUs, rows of at most C(14, 7) = 3432 entries, so that
+// four row-staged 256-thread blocks share a CU (LDS) against one at L = 16
+// (C(16, 8) = 12870 entries, 103 KB).  Measured (specialised H.v, ms, at
+// L = 12 / 13 / 14): 4x6 J1-J2 0.140 / 0.130 / 0.129; 2x12 0.155 / 0.146 /
+// 0.132; 4x8 47.3 / 42.4 / 39.4 (40.5 at 16); 6x6 772 / 717 at 13 / 14 (721
+// at 16); chains with J2 = 0.2: N = 24 0.0614 / 0.0605 / 0.0682, N = 26
+// 0.219 / 0.203 / 0.221, N = 28 0.827 / 0.882 at 13 / 14; N = 18 / 20 / 22
+// at L = 10: 9.7 / 11.6 / 22.3 us against 49 / 94 / 105 us at L = 16.
+// LSQED_SPLIT_L (8..16, keeping at most 24 high bits) overrides.
 int split_low_bits(const lsqed_ham* h, bool batch) {
   const int N = h->N;
   if (batch) {
@@ -557,7 +560,13 @@ int split_low_bits(const lsqed_ham* h, bool batch) {
     }
     return L;
   }
-  int L = N >= 20 ? std::max(std::max(10, std::min(14, N - 12)), N - 24) : std::min(N, 10);
+  int nb = 0;
+  for (int k = 0; k < 2; ++k)
+    for (size_t b = 0; b < h->bi[k].size(); ++b) nb += h->bi[k][b] != h->bj[k][b];
+  const bool dense = 2 * nb > 5 * N;
+  int L = std::min(N, 10);
+  if (N >= 20) L = dense ? std::max(10, std::min(14, N - 10)) : std::max(10, std::min(13, N - 11));
+  L = std::max(L, N - 24);
   if (const char* e = std::getenv("LSQED_SPLIT_L")) {
     const int v = std::atoi(e);
     if (v >= 8 && v <= 16 && v <= N && N - v <= 24) L = v;
