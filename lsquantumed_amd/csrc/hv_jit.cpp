@@ -43,6 +43,10 @@ static const char* kJitBody = R"JIT(
 #endif
 constexpr int kThreads = ROWSTAGE ? ROW_THREADS : 256;
 constexpr int kPerLane = 8;
+#ifndef GATHER_BATCH
+#define GATHER_BATCH 8
+#endif
+constexpr int kG = GATHER_BATCH;  // gathers issued together before any is used
 constexpr int kChunk = kThreads * kPerLane;
 
 #if WIDE
@@ -304,15 +308,15 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       const BT r = B + (BT)i;
       double acc = 0.0;
       // high bonds: row h ^ m, same entry; loads issued in batches of 8
-      for (int b0 = 0; b0 < nact; b0 += 8) {
-        double v[8];
+      for (int b0 = 0; b0 < nact; b0 += kG) {
+        double v[kG];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kG; ++u) {
           v[u] = 0.0;
           if (b0 + u < nact) v[u] = xv[rfl(s_hbase[pb][b0 + u]) + (BT)i];
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < kG; ++u)
           if (b0 + u < nact) acc += s_hcoef[pb][b0 + u] * v[u];
       }
       const unsigned int l = Uk[i];
@@ -325,11 +329,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       // index selection (non-flipping lanes read their own entry) so that the
       // gathers of a batch are all in flight together
 #pragma unroll
-      for (int b0 = 0; b0 < NCROSS; b0 += 8) {
-        double v[8];
+      for (int b0 = 0; b0 < NCROSS; b0 += kG) {
+        double v[kG];
         unsigned int onm = 0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kG; ++u) {
           const int b = b0 + u;
           if (b < NCROSS) {
             const int q = CRQ[b], hp = CRP[b], kind = CRK[b];
@@ -355,7 +359,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
           }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kG; ++u) {
           const int b = b0 + u;
           if (b < NCROSS) acc += (((onm >> u) & 1u) ? jh_[CRK[b]] : 0.0) * v[u];
         }
@@ -363,11 +367,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       }
       // low bonds: inside row h, entry RL(l ^ m)
 #pragma unroll
-      for (int b0 = 0; b0 < NLOW; b0 += 8) {
-        double v[8];
+      for (int b0 = 0; b0 < NLOW; b0 += kG) {
+        double v[kG];
         unsigned int onm = 0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kG; ++u) {
           const int b = b0 + u;
           if (b < NLOW) {
             const unsigned int m = LOWM[b];
@@ -398,7 +402,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
           }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kG; ++u) {
           const int b = b0 + u;
           if (b < NLOW) acc += (((onm >> u) & 1u) ? jh_[LOWK[b]] : 0.0) * v[u];
         }
@@ -583,6 +587,18 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   // occupancy hint of the kernel (waves per SIMD); LSQED_JIT_WAVES_PER_EU overrides the default 4
   if (const char* w = getenv("LSQED_JIT_WAVES_PER_EU"))
     if (atoi(w) >= 1 && atoi(w) <= 8) o << "#define WAVES_PER_EU " << atoi(w) << "\n";
+  // Gathers issued together: 16 for single-model kernels with 32-bit ranks
+  // and at least ~55 global gathers per entry (cross bonds plus about half
+  // the high bonds), else 8.  Results are bit-identical (the sums run in bond
+  // order either way).  H.v at 8 / 16: 4x8 38.9 / 37.2 ms, 8x4 35.2 / 33.5,
+  // 4x7 2.11 / 2.03, 2x12 0.132 / 0.128; 4x6 (49 gathers) 0.129 / 0.130;
+  // chain N=26 0.199 / 0.227; 6x6 (64-bit ranks) 686 / 691; the batched
+  // disorder solve 839 / 810 realisations/s.  LSQED_JIT_GATHER_BATCH
+  // (2..16) overrides.
+  int gather_batch = (!batch && N <= 32 && 2 * crq.size() + him.size() >= 110) ? 16 : 8;
+  if (const char* w = getenv("LSQED_JIT_GATHER_BATCH"))
+    if (atoi(w) >= 2 && atoi(w) <= 16) gather_batch = atoi(w);
+  o << "#define GATHER_BATCH " << gather_batch << "\n";
   if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
     if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
   if (jit_rowstage() && (!batch || L <= 14))
