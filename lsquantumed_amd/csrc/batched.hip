@@ -498,6 +498,12 @@ struct lsqed_batch {
   int ring = 0;
   hipStream_t gs = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // side streams (default priority) for the H.v of the smaller lattice
+  // groups, forked from and joined back to the launching stream
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kSide] = {nullptr, nullptr, nullptr};
   hipGraph_t graph[2] = {nullptr, nullptr};  // [0] plain steps, [1] steps + Ritz accumulation
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   // results
@@ -649,11 +655,46 @@ int launch_hv_batch(lsqed_batch* b, const double* q, double* y, hipStream_t s) {
     int64_t work = 0;
     for (const Group& g : b->groups) work += (int64_t)g.ntasks * (int64_t)g.idx.size();
     const int grid = grid_for(b, work);
-    for (const Group& g : b->groups) {
+    auto launch = [&](const Group& g, hipStream_t st) -> int {
       BatchJitLaunch L{q,           y,           b->d_hvpart + g.part0, b->d_poff, b->d_pcoup, b->d_pfields,
                        b->d_active, b->d_gidx + g.gofs, (int)g.idx.size()};
-      const int rc = launch_hv_batch_jit(g.module, g.h0, L, grid, s);
+      return launch_hv_batch_jit(g.module, g.h0, L, grid, st);
+    };
+    const int ng = (int)b->groups.size();
+    if (ng < 2 || !b->side[0]) {
+      for (const Group& g : b->groups) {
+        const int rc = launch(g, s);
+        if (rc) return rc;
+      }
+      return LSQED_OK;
+    }
+    // The groups write disjoint rows of y and of the partials: the smaller
+    // groups run on side streams beside the largest (a small group's launch
+    // alone leaves most CUs idle), all joined back before the step goes on.
+    // Inside a graph capture the fork and join become graph edges.
+    int big = 0;
+    for (int gi = 1; gi < ng; ++gi)
+      if ((int64_t)b->groups[gi].ntasks * (int64_t)b->groups[gi].idx.size() >
+          (int64_t)b->groups[big].ntasks * (int64_t)b->groups[big].idx.size())
+        big = gi;
+    LSQED_HIP(hipEventRecord(b->fork_ev, s));
+    int used = 0, k = 0;
+    for (int gi = 0; gi < ng; ++gi) {
+      if (gi == big) continue;
+      hipStream_t st = b->side[k % lsqed_batch::kSide];
+      if (k < lsqed_batch::kSide) {
+        LSQED_HIP(hipStreamWaitEvent(st, b->fork_ev, 0));
+        used = k + 1;
+      }
+      const int rc = launch(b->groups[gi], st);
       if (rc) return rc;
+      ++k;
+    }
+    const int rc = launch(b->groups[big], s);
+    if (rc) return rc;
+    for (int i = 0; i < used; ++i) {
+      LSQED_HIP(hipEventRecord(b->join_ev[i], b->side[i]));
+      LSQED_HIP(hipStreamWaitEvent(s, b->join_ev[i], 0));
     }
     return LSQED_OK;
   }
@@ -1001,10 +1042,17 @@ int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o,
   }
   if (hipStreamCreateWithFlags(&b->gs, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
     lsqed_batch_destroy(b);
     return fail(LSQED_ERR_HIP, "cannot create the batch stream");
   }
+  for (int i = 0; i < lsqed_batch::kSide; ++i)
+    if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming) != hipSuccess) {
+      lsqed_batch_destroy(b);
+      return fail(LSQED_ERR_HIP, "cannot create the batch side streams");
+    }
   *out = b;
   return LSQED_OK;
 }
@@ -1012,9 +1060,16 @@ int lsqed_batch_create(int n, lsqed_ham* const* hams, const lsqed_batch_opts* o,
 int lsqed_batch_destroy(lsqed_batch* b) {
   if (!b) return LSQED_OK;
   if (b->gs) (void)hipStreamSynchronize(b->gs);
+  for (hipStream_t st : b->side)
+    if (st) (void)hipStreamSynchronize(st);
   drop_graphs(b);
   for (hipEvent_t e : b->ev)
     if (e) (void)hipEventDestroy(e);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  for (hipEvent_t e : b->join_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t st : b->side)
+    if (st) (void)hipStreamDestroy(st);
   if (b->gs) (void)hipStreamDestroy(b->gs);
   for (void* ptr : {(void*)b->d_q, (void*)b->d_p, (void*)b->d_y, (void*)b->d_psi, (void*)b->d_part, (void*)b->d_s,
                     (void*)b->d_hist, (void*)b->d_obs, (void*)b->d_tasks, (void*)b->d_probs, (void*)b->d_active,
