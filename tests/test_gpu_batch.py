@@ -296,14 +296,17 @@ def test_disorder_average_driver_and_resume(O, torch, tmp_path):
 
 def test_batch_mixed_lattices_and_sectors(O, torch):
     """One batch holding a 3x4 square lattice (Sz = 0 and Sz = 1), a 6x2
-    honeycomb with J2 and a 10-site chain in Sz = -1, with and without
-    fields: each model its own specialised-kernel group; every E0 equals the
+    honeycomb with J2, a 10-site chain in Sz = -1, an 8-site chain and a 2x4
+    square lattice (doubled bonds), with and without fields: each model its
+    own specialised-kernel group (six groups, more than the three side
+    streams the smaller groups' launches share); every E0 equals the
     oracle's dense spectrum to 1e-11, for both H.v kernels."""
     import lsquantumed_amd as Q
     from lsquantumed_amd.batch import BatchSolver
     from test_gpu_hamiltonian import make
     specs = [("square", 12, 0, 1.0, 1.0, 0.5, 0.5, 3, 4, None), ("square", 12, 1, 1.0, 0.7, 0.0, 0.0, 3, 4, 0.3),
-             ("honeycomb", 12, 0, 1.0, 1.0, 0.4, 0.4, 6, 2, 0.8), ("chain", 10, -1, 1.0, 1.0, 0.2, 0.2, 0, 0, 1.5)]
+             ("honeycomb", 12, 0, 1.0, 1.0, 0.4, 0.4, 6, 2, 0.8), ("chain", 10, -1, 1.0, 1.0, 0.2, 0.2, 0, 0, 1.5),
+             ("chain", 8, 0, 1.0, 1.0, 0.0, 0.0, 0, 0, 0.5), ("square", 8, 0, 1.0, 0.5, 0.3, 0.3, 2, 4, None)]
     lt = {"chain": O.CHAIN, "square": O.SQUARE, "honeycomb": O.HONEYCOMB}
     hams, refs = [], []
     for k, (lat, N, mag, J1, D1, J2, D2, lx, ly, W) in enumerate(specs):
