@@ -529,11 +529,11 @@ static bool jit_rowstage() {
 }
 
 // Threads of a row-staged block: 256 for rows of at most C(14, 7) entries
-// (four blocks per CU), 1024 above (one block per CU holds a C(16, 8) row);
-// LSQED_JIT_ROW_THREADS (256, 512 or 1024) overrides.
+// (four blocks per CU), 512 for C(15, 7) (two), 1024 above (one block per CU
+// holds a C(16, 8) row); LSQED_JIT_ROW_THREADS (256, 512 or 1024) overrides.
 static int jit_row_threads(int L) {
   const char* e = getenv("LSQED_JIT_ROW_THREADS");
-  const int v = e ? atoi(e) : (L <= 14 ? 256 : 1024);
+  const int v = e ? atoi(e) : (L <= 14 ? 256 : (L == 15 ? 512 : 1024));
   return (v == 256 || v == 512) ? v : 1024;
 }
 
@@ -601,8 +601,8 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   o << "#define GATHER_BATCH " << gather_batch << "\n";
   if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
     if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
-  if (jit_rowstage() && (!batch || L <= 14))
-    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << (batch ? 256 : jit_row_threads(L))
+  if (jit_rowstage())
+    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L)
       << "\nconstexpr int RMAX = " << binom_u64(L, L / 2) << ";\n";
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";

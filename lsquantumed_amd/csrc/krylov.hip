@@ -728,14 +728,17 @@ int dcgs_update(Ctx& c, int j, const std::vector<double>& coef, double inv_nu, d
   return finish_reduce(c.e->h, c.e->part, G, 1, c.e->dsm, yy, c.s);
 }
 
-// Delayed CGS2 replaces the refined CGS2 on large sectors: two passes over
-// the basis per Krylov step instead of three.  Small sectors keep the
-// refined CGS2, whose rounding behaviour (which copies of a degenerate level
-// enter the space) the reference's tests pin.  LSQED_KS_DELAYED=0/1 forces
-// either form.
+// Delayed CGS2 replaces the refined CGS2 on sectors of 2^16 states or more:
+// two passes over the basis per Krylov step instead of three (and one host
+// synchronisation fewer).  Smaller sectors -- all those of the reference's
+// solver tests -- keep the refined CGS2 of SLEPc, whose rounding behaviour
+// (which copies of a degenerate level enter the space) those tests pin.
+// Measured: 4x8 nev=4 ncv=20 29.2 -> 25.0 s, 4x6 nev=10 ncv=32 0.144 ->
+// 0.115 s, the per-realisation loop over chains of 18-22 spins 76.5 -> 93.5
+// realisations/s.  LSQED_KS_DELAYED=0/1 forces either form.
 bool use_delayed(const lsqed_eps* e, int64_t D) {
   if (const char* v = std::getenv("LSQED_KS_DELAYED")) return std::atoi(v) == 1 && e->ncv <= 32;
-  return D >= ((int64_t)1 << 26) && e->ncv <= 32;
+  return D >= ((int64_t)1 << 16) && e->ncv <= 32;
 }
 
 // New random start vector for column k, orthonormal to V[:, 0:k].

@@ -182,6 +182,7 @@ def test_solver_errors(torch):
 
 def test_square4x6_j1j2_nev10(O, torch):
     """Periodic 4x6, J1=D1=1, J2=D2=0.5, Sz=0 (D = 2,704,156): Krylov-Schur
+    (the default orthogonalisation here: delayed CGS2)
     nev=10, ncv=32, tol 1e-9 against the oracle's spectrum (committed fixture,
     tests/golden/make_config2.py).  With SLEPc's stopping rule every returned
     value is an eigenvalue (1e-9 relative), the levels come in order and none
@@ -248,21 +249,23 @@ def test_square4x6_j1j2_nev10(O, torch):
     assert d_sisj <= 1e-8 and abs(m2 - fx["m2_af"]) <= 1e-8 and d_dsf <= 1e-8
 
 
-def test_delayed_cgs2_krylov_schur(O, torch, monkeypatch):
-    """Krylov-Schur with the delayed CGS2 (one projection pass V^T [u, A u]
-    and one update pass per step; the default on sectors of 2^26 states or
-    more) forced on smaller sectors with LSQED_KS_DELAYED=1.  Periodic 4x6
-    J1-J2, nev=10, ncv=32, tol 1e-9: every returned value is a level of the
-    oracle's spectrum (committed fixture) to 1e-9 relative, in order, with
-    EPS_ERROR_RELATIVE below 1e-8; with the multiplicity check (fresh
-    restart vectors) the 10 lowest with multiplicity to 1e-9.  Periodic chain
-    N=16: E0 = -7.142296360616785 (exact diagonalisation) to 1e-12, and the
-    same value through the refined CGS2."""
+def test_refined_and_delayed_cgs2(O, torch, monkeypatch):
+    """Both orthogonalisations of the Krylov-Schur solver on the same
+    sectors.  The default takes the delayed CGS2 (one projection pass
+    V^T [u, A u] and one update pass per step) on sectors of 2^16 states or
+    more -- test_square4x6_j1j2_nev10 runs it -- and SLEPc's refined CGS2
+    below.  Here LSQED_KS_DELAYED=0 forces the refined form on the periodic
+    4x6 J1-J2 sector: nev=10, ncv=32, tol 1e-9, every returned value a level
+    of the oracle's spectrum (committed fixture) to 1e-9 relative, in order,
+    with EPS_ERROR_RELATIVE below 1e-8; with the multiplicity check the 10
+    lowest with multiplicity to 1e-9.  LSQED_KS_DELAYED=1 forces the delayed
+    form on the periodic chain N=16 (12,870 states): E0 = -7.142296360616785
+    (exact diagonalisation) to 1e-12 with both forms."""
     import json
     import os
 
     import lsquantumed_amd as Q
-    monkeypatch.setenv("LSQED_KS_DELAYED", "1")
+    monkeypatch.setenv("LSQED_KS_DELAYED", "0")
     with open(os.path.join(os.path.dirname(__file__), "golden", "config2_square4x6_j1j2.json")) as f:
         fx = json.load(f)
     env = Q.Environment(24)
@@ -273,7 +276,7 @@ def test_delayed_cgs2_krylov_schur(O, torch, monkeypatch):
     nconv = Q.Solver.solve(env, eps)
     assert nconv >= 10
     got, errs = zip(*[Q.Solver.solution(eps, n) for n in range(10)])
-    say("delayed CGS2 4x6:", np.round(got, 9), "iterations", eps.iterations())
+    say("refined CGS2 4x6:", np.round(got, 9), "iterations", eps.iterations())
     Q.Solver.SolverClean(eps)
     assert max(errs) <= 1e-8, errs
     glev = [got[g[0]] for g in O.multiplets(np.array(got))]
@@ -285,12 +288,12 @@ def test_delayed_cgs2_krylov_schur(O, torch, monkeypatch):
     Q.Solver.SolverClean(eps)
     np.testing.assert_allclose(got, ref[:10], rtol=1e-9, atol=0)
     h16 = make("chain", 16, 0, 1.0, 1.0, 0.0, 0.0)
-    n1, v1, e1, _ = solve(h16, 1, tol=1e-12)
-    monkeypatch.setenv("LSQED_KS_DELAYED", "0")
     n0, v0, e0, _ = solve(h16, 1, tol=1e-12)
+    monkeypatch.setenv("LSQED_KS_DELAYED", "1")
+    n1, v1, e1, _ = solve(h16, 1, tol=1e-12)
     assert n1 >= 1 and n0 >= 1
-    assert abs(v1[0] + 7.142296360616785) <= 1e-12 * 7.2, v1[0]
-    assert abs(v1[0] - v0[0]) <= 1e-12 * 7.2, (v1[0], v0[0])
+    for v in (v0[0], v1[0]):
+        assert abs(v + 7.142296360616785) <= 1e-12 * 7.2, (v0[0], v1[0])
 
 
 def test_square4x8_ground_state_sum_rules(O, torch):
