@@ -182,6 +182,29 @@ def test_build_flags(O, torch):
     assert np.allclose(h.matvec(x).cpu().numpy(), m.csr(None, 2) @ x.cpu().numpy(), atol=1e-14)
 
 
+@pytest.mark.parametrize("lat,N,mag,lx,ly", [("chain", 38, -17, 0, 0), ("chain", 40, -18, 0, 0),
+                                              ("chain", 40, 18, 0, 0), ("square", 40, -17, 4, 10)])
+@pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
+def test_hv_largest_n_few_particle_sectors(O, torch, lat, N, mag, lx, ly, kernel):
+    """N = 38 and 40 in sectors of 2 or 3 up spins (or 2 down): the row split
+    at its widest -- L = N - 24 low sites (14, 16) under 24 high ones, 64-bit
+    ranks, 1024-thread row blocks at L = 16, rows that are mostly empty --
+    with J2 and fields, against the oracle's CSR to 1e-13."""
+    say("hv large N", lat, N, mag, kernel)
+    hi = np.random.default_rng(N + mag).uniform(-1, 1, N)
+    builds = ("build_diag", "build_off_diag", "build_disorder")
+    h = make(lat, N, mag, 1.0, 0.8, 0.4, 0.3, lx, ly, builds=builds, hi=hi)
+    h.set_hv_kernel(kernel)
+    m = oracle_model(O, lat, N, mag, 1.0, 0.8, 0.4, 0.3, lx, ly)
+    A = m.csr(hi, flags_of(O, builds))
+    D = h.hamilt_size()
+    assert D == A.shape[0]
+    x = np.random.default_rng(D).uniform(-1, 1, D)
+    y = h.matvec(torch.from_numpy(x).cuda()).cpu().numpy()
+    yref = A @ x
+    assert np.max(np.abs(y - yref)) <= 1e-13 * np.abs(A).max() * np.abs(x).max()
+
+
 @pytest.mark.parametrize("lat,N,lx,ly,J2", [("square", 24, 4, 6, 0.5), ("chain", 22, 0, 0, 0.2)])
 @pytest.mark.parametrize("kernel", [1, 2, 3], ids=["generic", "split", "split_jit"])
 def test_hv_config_sizes_matfree(O, torch, lat, N, lx, ly, J2, kernel):
