@@ -182,6 +182,49 @@ def test_build_flags(O, torch):
     assert np.allclose(h.matvec(x).cpu().numpy(), m.csr(None, 2) @ x.cpu().numpy(), atol=1e-14)
 
 
+@pytest.mark.parametrize("kernel", [1, 2, 3], ids=["generic", "split", "split_jit"])
+def test_hv_graph_capture_replays(O, torch, kernel):
+    """lsqed_hv captured into a HIP graph (torch.cuda.CUDAGraph, on the
+    capturing stream) and replayed three times, with eager work and
+    synchronisations between the replays: every replay equals the eager
+    product bit for bit, also after new contents are copied into x (the
+    graph holds the pointers).  4x5 J1-J2 with fields (D = 184,756)."""
+    say("hv graph capture", kernel)
+    N = 20
+    hi = np.random.default_rng(7).uniform(-1, 1, N)
+    builds = ("build_diag", "build_off_diag", "build_disorder")
+    h = make("square", N, 0, 1.0, 1.0, 0.5, 0.5, 4, 5, builds=builds, hi=hi)
+    h.set_hv_kernel(kernel)
+    D = h.hamilt_size()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(D, dtype=torch.float64, device="cuda", generator=gen)
+    x2 = torch.rand(D, dtype=torch.float64, device="cuda", generator=gen)
+    ref = h.matvec(x)  # eager; also builds the tables and compiles the kernel
+    ref2 = h.matvec(x2)
+    y = torch.empty_like(x)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        h.matvec(x, y)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        h.matvec(x, y)
+    for rep in range(3):
+        y.zero_()
+        junk = torch.rand(D, dtype=torch.float64, device="cuda", generator=gen) * 2.0  # eager work in between
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref), (kernel, rep)
+        del junk
+    x.copy_(x2)
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(y, ref2), kernel
+
+
 @pytest.mark.parametrize("lat,N,mag,lx,ly", [("chain", 38, -17, 0, 0), ("chain", 40, -18, 0, 0),
                                               ("chain", 40, 18, 0, 0), ("square", 40, -17, 4, 10)])
 @pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
