@@ -534,8 +534,9 @@ int split_tables(lsqed_ham* h, SplitTables& t, bool batch) {
 
 // Low bits of the row split.  The batched kernel takes L = min(14, N) (its
 // rows are staged in LDS like the single-model kernel's; many problems give
-// the parallelism; LSQED_BATCH_SPLIT_L, 8..16, overrides -- above 14 it uses
-// 2048-entry work items without staging).  The single-model kernels (one
+// the parallelism; LSQED_BATCH_SPLIT_L, 8..16, overrides: on the 256-chain
+// disorder batch L = 11 / 12 / 13 / 14 / 15 / 16 gave 733 / 840 / 866 / 940 /
+// 835 / 793 realisations/s).  The single-model kernels (one
 // row-staged block per row) take, for N >= 20, L = clamp(N - 10, 10, 14) on
 // lattices with more than 2.5 flippable bonds per site and clamp(N - 11, 10,
 // 13) on sparser ones (a further low site moves only ~2 bonds into LDS there
