@@ -384,7 +384,7 @@ def autograd_chain20(B: int = 64, N: int = 20):
         fj2 = (ground_energy(hh, 1.0, 0.3 + d).sum() - ground_energy(hh, 1.0, 0.3 - d).sum()).item() / (2 * d)
     return {"workload": f"E0(h, J1, J2) and its gradient, {B} periodic chains of N={N}, J1=1, J2=0.3, "
                         "h_i~U[-1,1] from mt19937_64(1+k)",
-            "forward_backward_s": round(wall, 4), "E0_mean": float(E.mean()),
+            "forward_backward_s": round(wall, 4), "E0_mean": float(E.detach().mean()),
             "dE_dJ1": gj1, "dE_dJ1_fd": fj1, "dE_dJ1_rel_err": abs(gj1 - fj1) / abs(gj1),
             "dE_dJ2": gj2, "dE_dJ2_fd": fj2, "dE_dJ2_rel_err": abs(gj2 - fj2) / abs(gj2),
             "sum_dE_dh": float(gh.sum())}
