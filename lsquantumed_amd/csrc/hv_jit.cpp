@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <memory>
 #include <mutex>
 #include <sstream>
@@ -537,8 +538,10 @@ static int jit_row_threads(int L) {
   return (v == 256 || v == 512) ? v : 1024;
 }
 
+// Compiled modules by (device, generated source): a module is loaded into one
+// device's context and cannot launch on another.
 static std::mutex g_jit_mu;
-static std::map<std::string, std::shared_ptr<JitModule>> g_jit_cache;
+static std::map<std::pair<int, std::string>, std::shared_ptr<JitModule>> g_jit_cache;
 
 template <typename T>
 static void emit_array(std::ostringstream& o, const char* name, const std::vector<T>& v, const char* type) {
@@ -682,12 +685,13 @@ int jit_prepare(lsqed_ham* h) {
   std::shared_ptr<JitModule> m;
   {
     std::lock_guard<std::mutex> lk(g_jit_mu);
-    auto it = g_jit_cache.find(src);
+    const auto keyd = std::make_pair(h->device, src);
+    auto it = g_jit_cache.find(keyd);
     if (it != g_jit_cache.end()) {
       m = it->second;
     } else {
       m = jit_compile(src, false);
-      g_jit_cache[src] = m;
+      g_jit_cache[keyd] = m;
     }
   }
   if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the specialised H.v failed: " + m->log);
@@ -756,12 +760,13 @@ int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key) {
   std::shared_ptr<JitModule> m;
   {
     std::lock_guard<std::mutex> lk(g_jit_mu);
-    auto it = g_jit_cache.find(src);
+    const auto keyd = std::make_pair(h->device, src);
+    auto it = g_jit_cache.find(keyd);
     if (it != g_jit_cache.end()) {
       m = it->second;
     } else {
       m = jit_compile(src, true);
-      g_jit_cache[src] = m;
+      g_jit_cache[keyd] = m;
     }
   }
   if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the batched H.v failed: " + m->log);
