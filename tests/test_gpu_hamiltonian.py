@@ -248,11 +248,13 @@ def test_hv_largest_n_few_particle_sectors(O, torch, lat, N, mag, lx, ly, kernel
     assert np.max(np.abs(y - yref)) <= 1e-13 * np.abs(A).max() * np.abs(x).max()
 
 
-@pytest.mark.parametrize("lat,N,lx,ly,J2", [("square", 24, 4, 6, 0.5), ("chain", 22, 0, 0, 0.2)])
+@pytest.mark.parametrize("lat,N,lx,ly,J2", [("square", 24, 4, 6, 0.5), ("chain", 22, 0, 0, 0.2),
+                                             ("honeycomb", 24, 4, 6, 0.4)])
 @pytest.mark.parametrize("kernel", [1, 2, 3], ids=["generic", "split", "split_jit"])
 def test_hv_config_sizes_matfree(O, torch, lat, N, lx, ly, J2, kernel):
-    """Full-size H.v of the 4x6 J1-J2 lattice and the N=22 chain against the
-    oracle's matrix-free product (same row semantics, exact binomials)."""
+    """Full-size H.v of the 4x6 J1-J2 lattice, the N=22 chain and the 4x6
+    honeycomb (24 sites, D = 2,704,156) against the oracle's matrix-free
+    product (same row semantics, exact binomials)."""
     say("hv config", lat, N, kernel)
     h = make(lat, N, 0, 1.0, 1.0, J2, J2, lx, ly)
     h.set_hv_kernel(kernel)
@@ -303,6 +305,30 @@ def formula_vector(O, torch, D, chunk=1 << 27):
         x[s:s + c.numel()] = n.double() * (1.0 / 1048576.0) - 1.0
         del c, n
     return x
+
+
+@pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
+def test_hv_honeycomb32_fields_rows_formula_vector(O, torch, kernel):
+    """4x8 honeycomb (N = 32) with J2 and random fields in Sz = 0 and Sz = 1
+    (D = 601,080,390 and 565,722,720): rows of H x sampled across each
+    sector against the oracle, for the oracle's formula vector."""
+    builds = ("build_diag", "build_off_diag", "build_disorder")
+    hi = np.random.default_rng(32).uniform(-2, 2, 32)
+    for mag in (0, 1):
+        h = make("honeycomb", 32, mag, 1.0, 0.7, 0.4, 0.3, 4, 8, builds=builds, hi=hi)
+        h.set_hv_kernel(kernel)
+        D = h.hamilt_size()
+        assert D == math.comb(32, 16 + mag)
+        say("hv honeycomb 4x8 Sz", mag, "kernel", h.hv_kernel())
+        x = formula_vector(O, torch, D)
+        y = h.matvec(x)
+        m = oracle_model(O, "honeycomb", 32, mag, 1.0, 0.7, 0.4, 0.3, 4, 8)
+        for row0 in (0, 98765432, D // 2, D - 3000):
+            yref = m.hv_rows_formula(row0, 3000, hi=hi)
+            got = y[row0:row0 + 3000].cpu().numpy()
+            assert np.max(np.abs(got - yref)) <= 1e-12 * np.abs(yref).max(), (mag, row0)
+        del x, y
+        torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])

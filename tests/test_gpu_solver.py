@@ -197,6 +197,31 @@ def test_lanczos_ground_matches_krylov_schur(O, torch):
     assert abs(e0 - vals[0]) <= 1e-10 * abs(vals[0]), (e0, vals[0], steps)
 
 
+def test_honeycomb_spectra_with_fields(O, torch):
+    """Honeycomb lattices (honeycomb2D, src/lattice.cpp:140-264), which the
+    reference's solver tests do not cover, with J2 and random fields (no
+    degenerate levels): 4x4 (N = 16) in Sz = 0 and Sz = 1, nev = 4, against
+    scipy's eigsh on the oracle's CSR; 6x2 (N = 12), nev = 6, against the
+    dense spectrum.  Values to 1e-10 relative, in order, with
+    EPS_ERROR_RELATIVE below 1e-8."""
+    from scipy.sparse.linalg import eigsh
+    builds = ("build_diag", "build_off_diag", "build_disorder")
+    for N, mag, lx, ly, nev in ((16, 0, 4, 4, 4), (16, 1, 4, 4, 4), (12, 0, 6, 2, 6)):
+        say("honeycomb spectrum", N, mag, lx, ly)
+        hi = np.random.default_rng(100 + N + mag).uniform(-1, 1, N)
+        h = make("honeycomb", N, mag, 1.0, 1.0, 0.4, 0.4, lx, ly, builds=builds, hi=hi)
+        m = oracle_model(O, "honeycomb", N, mag, 1.0, 1.0, 0.4, 0.4, lx, ly)
+        A = m.csr(hi, flags_of(O, builds))
+        if A.shape[0] <= 2000:
+            ref = np.linalg.eigvalsh(A.toarray())[:nev]
+        else:
+            ref = np.sort(eigsh(A, k=nev, which="SA", tol=1e-14)[0])
+        nconv, vals, errs, _ = solve(h, nev, tol=1e-12)
+        assert nconv >= nev, (N, mag, nconv)
+        np.testing.assert_allclose(vals[:nev], ref, rtol=1e-10, atol=0)
+        assert max(errs[:nev]) <= 1e-8, errs
+
+
 def test_solver_errors(torch):
     import lsquantumed_amd as Q
     h = make("chain", 6, 0)
