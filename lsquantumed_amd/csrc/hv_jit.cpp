@@ -173,7 +173,10 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   // XCD b % 8 (the dispatcher deals blocks round-robin over the XCDs) and
   // takes rows b / 8, b / 8 + P, ... of its segment (P blocks per XCD; one
   // row per block when the grid covers the list), so the blocks of one XCD
-  // work on adjacent rows at a time and meet in its L2.
+  // work on adjacent rows at a time and meet in its L2.  Giving a block
+  // adjacent rows instead (rows jK .. jK+K-1) measured slower at two rows
+  // per block (4x8 37.8 vs 37.2-37.4 ms, 6x6 700 vs 686 ms): neighbouring
+  // rows have correlated lengths, so a block's pair is less often balanced.
   const int xcd = blockIdx.x & 7, P = (int)(G >> 3), seg = a.ntasks >> 3;
   double dotacc = 0.0;
   int pb = 0;
@@ -518,8 +521,8 @@ struct JitModule {
   std::string log;
 };
 
-// The single-model kernel stages each row of x in LDS (persistent 1024-thread
-// blocks walking whole rows) unless LSQED_JIT_ROWSTAGE=0.  The low bonds then read LDS instead
+// The single-model kernel stages each row of x in LDS (blocks of 256-1024
+// threads, each taking whole rows) unless LSQED_JIT_ROWSTAGE=0.  The low bonds then read LDS instead
 // of issuing global gathers: 37% fewer vector memory instructions on the
 // 4x8 J1-J2 sector, whose H.v was bound by address processing (TA busy 92%);
 // 43.7 -> 41.0 ms there, 779 -> 736 ms on 6x6, 0.185 -> 0.152 ms on 4x6,
