@@ -10,6 +10,8 @@ import sys
 import numpy as np
 import pytest
 
+from closed_forms import CLOSED, magnon_levels, polarised_energy
+
 pytestmark = pytest.mark.gpu
 
 LT = {"chain": 0, "square": 1, "honeycomb": 2}
@@ -246,6 +248,39 @@ def test_hv_largest_n_few_particle_sectors(O, torch, lat, N, mag, lx, ly, kernel
     y = h.matvec(torch.from_numpy(x).cuda()).cpu().numpy()
     yref = A @ x
     assert np.max(np.abs(y - yref)) <= 1e-13 * np.abs(A).max() * np.abs(x).max()
+
+
+@pytest.mark.parametrize("lat,N,lx,ly", CLOSED, ids=[f"{c[0]}{c[1]}" for c in CLOSED])
+def test_hv_closed_form_sectors(torch, lat, N, lx, ly):
+    """Sectors whose spectrum is known in closed form, through every H.v
+    kernel that takes N (the unspecialised row split stops at 32): the
+    fully polarised sectors with fields (D = 1, polarised_energy) and the
+    single-magnon sectors n_up = 1 and N - 1 (D = N; the dense H assembled
+    column by column from H e_i has the spectrum of magnon_levels to
+    1e-12)."""
+    J1, D1, J2, D2 = 1.0, 0.8, 0.4, 0.3
+    hi = np.random.default_rng(N).uniform(-1, 1, N)
+    fb = ("build_diag", "build_off_diag", "build_disorder")
+    for kernel in ([1, 2, 3] if N <= 32 else [1, 3]):
+        for mag in (N // 2, -(N // 2)):
+            say("closed form: polarised", lat, N, mag, kernel)
+            h = make(lat, N, mag, J1, D1, J2, D2, lx, ly, builds=fb, hi=hi)
+            h.set_hv_kernel(kernel)
+            assert h.hamilt_size() == 1
+            e = polarised_energy(lat, N, J1, D1, J2, D2, hi, up=mag > 0)
+            y = h.matvec(torch.ones(1, dtype=torch.float64, device="cuda")).item()
+            assert abs(y - e) <= 1e-14 * max(1.0, abs(e)), (y, e)
+            assert abs(h.diagonal().item() - e) <= 1e-14 * max(1.0, abs(e))
+        ref = magnon_levels(lat, N, lx, ly, J1, D1, J2, D2)
+        for mag in (1 - N // 2, N // 2 - 1):
+            say("closed form: magnon", lat, N, mag, kernel)
+            h = make(lat, N, mag, J1, D1, J2, D2, lx, ly)
+            h.set_hv_kernel(kernel)
+            assert h.hamilt_size() == N
+            eye = torch.eye(N, dtype=torch.float64, device="cuda")
+            H = torch.stack([h.matvec(eye[i].contiguous()) for i in range(N)], dim=1).cpu().numpy()
+            assert np.array_equal(H, H.T), (lat, N, mag, kernel)
+            np.testing.assert_allclose(np.linalg.eigvalsh(H), ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
 @pytest.mark.parametrize("lat,N,lx,ly,J2", [("square", 24, 4, 6, 0.5), ("chain", 22, 0, 0, 0.2),

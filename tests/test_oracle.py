@@ -12,6 +12,8 @@ import math
 import numpy as np
 import pytest
 
+from closed_forms import CLOSED, magnon_levels, polarised_energy
+
 LT = {"chain": 0, "square": 1, "honeycomb": 2}
 
 
@@ -219,6 +221,25 @@ def test_xx_chain_closed_form(O):
     assert abs(w[0] - eps[:5].sum()) < 1e-12
     assert abs(w[0] + 1 - np.sqrt(2)) > 0  # the literal is -(1+sqrt2)
     assert abs(w[0] - (-(1 + np.sqrt(2)))) < 1e-13
+
+
+@pytest.mark.parametrize("lat,N,lx,ly", CLOSED, ids=[f"{c[0]}{c[1]}" for c in CLOSED])
+def test_closed_form_sectors(O, lat, N, lx, ly):
+    """The oracle's H up to N = 40 against closed forms: the fully polarised
+    sectors with fields (one state) and the spectra of the single-magnon
+    sectors n_up = 1 and N - 1 (to 1e-12)."""
+    J1, D1, J2, D2 = 1.0, 0.8, 0.4, 0.3
+    lt = {"chain": O.CHAIN, "square": O.SQUARE}[lat]
+    hi = np.random.default_rng(N).uniform(-1, 1, N)
+    full = O.Model.BUILD_DIAG | O.Model.BUILD_OFFDIAG | O.Model.BUILD_DISORDER
+    for nup in (N, 0):
+        A = O.Model(lt, N, nup, J1, D1, J2, D2, lx=lx, ly=ly).csr(hi, full).toarray()
+        e = polarised_energy(lat, N, J1, D1, J2, D2, hi, up=nup == N)
+        assert A.shape == (1, 1) and abs(A[0, 0] - e) <= 1e-14 * max(1.0, abs(e)), (nup, A, e)
+    ref = magnon_levels(lat, N, lx, ly, J1, D1, J2, D2)
+    for nup in (1, N - 1):
+        A = O.Model(lt, N, nup, J1, D1, J2, D2, lx=lx, ly=ly).csr().toarray()
+        np.testing.assert_allclose(np.linalg.eigvalsh(A), ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
 def test_ising_limit(O):
