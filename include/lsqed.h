@@ -164,8 +164,11 @@ typedef struct lsqed_eps lsqed_eps;
 
 /* Solver::SolverInit + Solver::solve (src/solver.cpp:7-52): device-resident
  * thick-restart Lanczos (Krylov-Schur for a symmetric operator) with full
- * CGS2 reorthogonalisation; convergence when ||r_i|| / |theta_i| < tol
- * (EPS_CONV_REL).  Allocates (ncv+1) D-vectors of workspace. */
+ * CGS2 reorthogonalisation -- SLEPc's refined form (three passes over the
+ * basis per step) below 2^16 states, a delayed form (two passes) from 2^16
+ * states when ncv <= 32 (LSQED_KS_DELAYED=0/1 forces either); convergence
+ * when ||r_i|| / |theta_i| < tol (EPS_CONV_REL).  Allocates (ncv+1)
+ * D-vectors of workspace. */
 int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out);
 int lsqed_eps_solve(lsqed_eps* e, int* nconv, lsqed_stream_t stream);
 /* Solver::solution (src/solver.cpp:58-68): eigenpair i (ascending) and the

@@ -39,6 +39,15 @@ def correlations(h, psi, stream=None) -> dict:
     return {"SzSz": zz, "SpSm": pm, "SmSp": mp, "SiSj": sisj, "Sz": sz}
 
 
+def magnetisation(h, psi, stream=None) -> np.ndarray:
+    """<S^z_i> for every site i of the real state psi (CUDA float64, length D)
+    in one device pass: sum_s |psi_s|^2 (bit_i(s) - 1/2), the diagonal of
+    SzSz/SpSm reduced to one site (correlation.cpp:86-139)."""
+    out = np.zeros(h.basis.nspins)
+    check(lib().lsqed_obs_sz(h.hamilt, ptr(psi), out.ctypes.data_as(C.c_void_p), stream_ptr(stream)))
+    return out
+
+
 class _Corr:
     key = ""
 

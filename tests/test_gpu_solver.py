@@ -120,8 +120,9 @@ def test_correlation_literals(O, torch, literals):
 
 
 def test_all_pair_correlators_vs_oracle(O, torch):
-    """Every (i, j) of SzSz, SpSm, SmSp, SiSj and <S^z_i> with random fields
-    (so that <S^z_i> != 0), 3x4 lattice with J2, mag 1."""
+    """Every (i, j) of SzSz, SpSm, SmSp, SiSj and <S^z_i> (from the pair pass
+    and from lsqed_obs_sz) with random fields (so that <S^z_i> != 0), 3x4
+    lattice with J2, mag 1."""
     say("all pairs")
     hi = np.random.default_rng(1).uniform(-1, 1, 12)
     h = make("square", 12, 1, 1.0, 1.0, 0.4, 0.4, 3, 4, builds=("build_diag", "build_off_diag", "build_disorder"),
@@ -140,6 +141,10 @@ def test_all_pair_correlators_vs_oracle(O, torch):
     sz_ref = [0.5 * np.sum(psi ** 2 * np.where((O.int_basis(12, nup) >> np.uint64(i)) & np.uint64(1), 1, -1))
               for i in range(12)]
     assert np.max(np.abs(c["Sz"] - np.array(sz_ref))) < 1e-12
+    from lsquantumed_amd import lib, magnetisation
+    assert np.max(np.abs(magnetisation(h, vecs[0]) - np.array(sz_ref))) < 1e-12
+    # the model's recorded sector, through the C-ABI accessors
+    assert lib().lsqed_ham_nspins(h.hamilt) == 12 and lib().lsqed_ham_nup(h.hamilt) == nup
 
 
 def test_magnetisation_vs_oracle(O, torch):
