@@ -189,6 +189,22 @@ def test_batch_errors(torch):
         BatchSolver([big])
 
 
+def test_batch_workspace_bytes(torch):
+    """lsqed_batch_workspace_bytes: 3 doubles per basis state of every model
+    (4 with the ground-state vectors kept), -1 for an empty batch."""
+    import ctypes as C
+
+    from lsquantumed_amd import lib
+    from lsquantumed_amd._lib import BatchOpts
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    hams = disorder_chains([8, 10, 12], [2, 1, 3], J1=1.0, J2=0.2, W=1.0)
+    states = 2 * 70 + 252 + 3 * 924
+    assert sum(h.hamilt_size() for h in hams) == states
+    assert BatchSolver.workspace_bytes(hams) == 3 * 8 * states
+    assert BatchSolver.workspace_bytes(hams, keep_vectors=True) == 4 * 8 * states
+    assert lib().lsqed_batch_workspace_bytes(0, None, C.byref(BatchOpts(0, 0.0, 0, 0, 0, 0))) == -1
+
+
 def _fd_check(torch, B, N, J1v, J2v, seed, fd_rows):
     from lsquantumed_amd.autograd import ground_energy
     from lsquantumed_amd.tools import fields_from_seed

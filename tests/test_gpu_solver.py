@@ -227,6 +227,78 @@ def test_honeycomb_spectra_with_fields(O, torch):
         assert max(errs[:nev]) <= 1e-8, errs
 
 
+def test_lanczos_step_recurrence(O, torch):
+    """lsqed_lanczos_step (the unit of the steps/s measurement), driven as the
+    three-vector loop drives it (q and p swapped after each step, their
+    scale factors carried on the host): alpha_m and beta_m equal the
+    textbook three-term recurrence on the oracle's CSR (numpy, same start
+    vector) to 1e-10 relative over the first 12 steps, and after 80 steps
+    the lowest Ritz value is the oracle's lowest eigenvalue to 1e-10.  4x4
+    J1-J2 with fields (D = 12,870)."""
+    import scipy.linalg as sl
+    from scipy.sparse.linalg import eigsh
+
+    from lsquantumed_amd import lanczos_step
+    builds = ("build_diag", "build_off_diag", "build_disorder")
+    hi = np.random.default_rng(16).uniform(-1, 1, 16)
+    h = make("square", 16, 0, 1.0, 1.0, 0.5, 0.5, 4, 4, builds=builds, hi=hi)
+    A = oracle_model(O, "square", 16, 0, 1.0, 1.0, 0.5, 0.5, 4, 4).csr(hi, flags_of(O, builds))
+    D = A.shape[0]
+    x0 = np.random.default_rng(11).uniform(-1, 1, D)
+    v, vp, b = x0 / np.linalg.norm(x0), np.zeros(D), 0.0
+    ra, rb = [], []
+    for _ in range(12):
+        w = A @ v
+        a = float(v @ w)
+        w = w - a * v - b * vp
+        b = float(np.linalg.norm(w))
+        vp, v = v, w / b
+        ra.append(a)
+        rb.append(b)
+    q = torch.from_numpy(x0).cuda()
+    p = torch.zeros(D, dtype=torch.float64, device="cuda")
+    y = torch.empty(D, dtype=torch.float64, device="cuda")
+    sq, sp, beta = 1.0 / float(np.linalg.norm(x0)), 0.0, 0.0
+    al, be = [], []
+    for _ in range(80):
+        a, b = lanczos_step(h, q, sq, p, sp, beta, y)
+        al.append(a)
+        be.append(b)
+        q, p = p, q
+        sp, sq, beta = sq, 1.0 / b, b
+    say("lanczos_step alpha", al[:3], "beta", be[:3])
+    np.testing.assert_allclose(al[:12], ra, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(be[:12], rb, rtol=1e-10)
+    theta = sl.eigh_tridiagonal(np.array(al), np.array(be[:-1]), eigvals_only=True, select="i",
+                                select_range=(0, 0))[0]
+    e0 = float(eigsh(A, k=1, which="SA", tol=1e-14)[0][0])
+    assert abs(theta - e0) <= 1e-10 * abs(e0), (theta, e0)
+
+
+def test_stream_probe(torch):
+    """lsqed_stream_probe, the measured streaming ceilings: the triad writes
+    exactly b + 0.5 c, the read stream's 2048 block sums add up to sum(b);
+    odd n for the read stream and an unknown kind are argument errors."""
+    from lsquantumed_amd import lib
+    from lsquantumed_amd._lib import ptr
+    n = 1_000_002
+    g = torch.Generator(device="cuda").manual_seed(5)
+    b = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+    c = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+    a = torch.empty_like(b)
+    part = torch.zeros(2048, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    assert lib().lsqed_stream_probe(0, ptr(a), ptr(b), ptr(c), n, None, None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(a, b + 0.5 * c)
+    assert lib().lsqed_stream_probe(1, None, ptr(b), None, n, ptr(part), None) == 0
+    torch.cuda.synchronize()
+    tot = b.sum().item()
+    assert abs(part.sum().item() - tot) <= 1e-12 * tot
+    assert lib().lsqed_stream_probe(1, None, ptr(b), None, n - 1, ptr(part), None) == -1
+    assert lib().lsqed_stream_probe(2, ptr(a), ptr(b), ptr(c), n, None, None) == -1
+
+
 def test_solver_errors(torch):
     import lsquantumed_amd as Q
     h = make("chain", 6, 0)
