@@ -588,6 +588,8 @@ def main():
                     help="only the batched-disorder measurement (one JSON object; used for profiling)")
     ap.add_argument("--lanczos-6x6", action="store_true",
                     help="only the 6x6 (N=36) three-vector Lanczos ground state (218 GB; minutes)")
+    ap.add_argument("--maxit", type=int, default=600,
+                    help="with --lanczos-6x6: at most this many Lanczos steps (a few for a profile)")
     ap.add_argument("--krylov-4x8", action="store_true",
                     help="only Krylov-Schur nev=4 ncv=20 on the 4x8 sector (101 GB basis; about a minute)")
     ap.add_argument("--hv-only", action="store_true",
@@ -608,7 +610,7 @@ def main():
         print(json.dumps({"batched_disorder": disorder_batch()}))
         return
     if args.lanczos_6x6:
-        print(json.dumps({"lanczos_6x6": lanczos_6x6()}))
+        print(json.dumps({"lanczos_6x6": lanczos_6x6(maxit=max(1, args.maxit))}))
         return
     if args.krylov_4x8:
         print(json.dumps({"krylov_schur_4x8": krylov_schur_4x8()}))

@@ -32,7 +32,7 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-// Sum over a 256-thread block, valid in thread 0.
+// Sum over a block of at most 512 threads, valid in thread 0.
 __device__ __forceinline__ double bsum(double v) {
   __shared__ double red[8];
   v = wsum(v);
@@ -79,11 +79,23 @@ __global__ void k_dot(const double* __restrict__ x, const double* __restrict__ y
 }
 
 // out[c] = sum_i part[c*n + i]  (one block per column, fixed order)
-__global__ void k_reduce_cols(const double* __restrict__ part, int n, double* __restrict__ out) {
+// Column sums of partials (n per column), one 512-thread block per column.
+// Eight independent accumulators per lane keep eight loads in flight: the
+// x.y partials of a large H.v launch (one per block) number 2^17 on 4x8 and
+// 2^21 on 6x6, which one load at a time per lane took 3.4 ms to sum on 6x6.
+// The order of the sums is fixed, so results are deterministic.
+constexpr int kRedColThreads = 512;
+__global__ __launch_bounds__(kRedColThreads) void k_reduce_cols(const double* __restrict__ part, int n,
+                                                                double* __restrict__ out) {
   const double* p = part + (int64_t)blockIdx.x * n;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += p[i];
-  double t = bsum(acc);
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 7 * kRedColThreads < n; i += 8 * kRedColThreads) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += p[i + u * kRedColThreads];
+  }
+  for (; i < n; i += kRedColThreads) acc[0] += p[i];
+  const double t = bsum(((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
   if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
@@ -396,7 +408,7 @@ static int stream_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<in
 // Sum `ncol` columns of partials (n each) to host.
 static int finish_reduce(lsqed_ham* h, const double* part, int n, int ncol, double* dev_out, double* host_out,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_cols, dim3(ncol), dim3(kThreads), 0, s, part, n, dev_out);
+  hipLaunchKernelGGL(k_reduce_cols, dim3(ncol), dim3(kRedColThreads), 0, s, part, n, dev_out);
   LSQED_HIP(hipGetLastError());
   LSQED_HIP(hipMemcpyAsync(host_out, dev_out, ncol * sizeof(double), hipMemcpyDeviceToHost, s));
   LSQED_HIP(hipStreamSynchronize(s));
