@@ -136,7 +136,9 @@ int lsqed_export_cols(lsqed_ham* h, int kind, int64_t row0, int64_t nrows, int m
 
 /* Unrank rows [row0, row0+n) to states (BasisHelper::search_elem +
  * Basis::generate_int_basis, src/basis.cpp:9-36, :192-215) and rank states
- * (HamiltHelper::search_index, src/hamiltonian.cpp:9-27).  Device buffers. */
+ * (HamiltHelper::search_index, src/hamiltonian.cpp:9-27).  Device buffers.
+ * lsqed_rank writes -1 for a state outside the handle's sector (a bit set at
+ * or above N, or a number of set bits other than nup). */
 int lsqed_unrank(lsqed_ham* h, int64_t row0, int64_t n, uint64_t* states, lsqed_stream_t stream);
 int lsqed_rank(lsqed_ham* h, const uint64_t* states, int64_t n, int64_t* ranks, lsqed_stream_t stream);
 

@@ -247,8 +247,13 @@ __global__ void unrank_kernel(RowParams p, const R* __restrict__ binom, int64_t 
 template <typename S, typename R, int W, int NC>
 __global__ void rank_kernel(RowParams p, const R* __restrict__ tab, const uint64_t* __restrict__ st, int64_t n,
                             int64_t* __restrict__ out) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = (int64_t)rank_state<S, R, W, NC>((S)st[i], tab, p);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = st[i];
+    // A state outside the sector (a bit at or above N, or not nup set bits)
+    // has no rank; it would index table rows outside [kmin, kmax].
+    const bool in_sector = (p.N >= 64 || (s >> p.N) == 0) && __popcll(s) == p.nup;
+    out[i] = in_sector ? (int64_t)rank_state<S, R, W, NC>((S)s, tab, p) : (int64_t)-1;
+  }
 }
 
 // -------------------------------------------------------------- launchers --

@@ -101,3 +101,52 @@ def lanczos_step(h, q, sq, p, sp, beta, y, stream=None):
     check(lib().lsqed_lanczos_step(h.hamilt, ptr(q), float(sq), ptr(p), float(sp), float(beta), ptr(y), C.byref(a),
                                    C.byref(b), stream_ptr(stream)))
     return a.value, b.value
+
+
+def lanczos_ground_residual(h, tol=1e-9, maxit=600, seed=7, check_every=5, progress=None, stream=None):
+    """Ground energy by the three-vector Lanczos (q, p, y on the device, the
+    tridiagonal matrix on the host; no reorthogonalisation, no eigenvector),
+    stopped when the lowest Ritz pair (theta, s) has the relative residual
+    |beta_m s_m| / |theta| < tol -- the EPS_CONV_REL test of the Krylov-Schur
+    solver (src/solver.cpp:38-52) -- checked every `check_every` steps.  For
+    sectors whose Krylov-Schur basis does not fit (N = 36: 72.6 GB per
+    vector).  Start vector uniform in [-0.5, 0.5) from torch's device
+    generator seeded with `seed`.  progress(step, theta, resid, seconds) is
+    called at every check.  Returns a dict with E0, steps, the residual and
+    the wall time of the iteration."""
+    import time
+
+    import numpy as np
+    import scipy.linalg as sl
+    import torch
+
+    D = h.hamilt_size()
+    g = torch.Generator(device="cuda").manual_seed(int(seed))
+    q = torch.rand(D, dtype=torch.float64, device="cuda", generator=g)
+    q -= 0.5
+    p = torch.zeros(D, dtype=torch.float64, device="cuda")
+    y = torch.empty(D, dtype=torch.float64, device="cuda")
+    ss = sum(float(torch.dot(q[i:i + 2**30], q[i:i + 2**30]).item()) for i in range(0, D, 2**30))
+    sq, sp, beta = 1.0 / ss ** 0.5, 0.0, 0.0
+    al, be = [], []
+    theta, res = None, None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in range(int(maxit)):
+        a, b = lanczos_step(h, q, sq, p, sp, beta, y, stream)
+        al.append(a)
+        be.append(b)
+        exhausted = len(al) >= D or not b > 1e-14 * abs(a)  # invariant subspace reached
+        if not exhausted:
+            q, p = p, q
+            sp, sq, beta = sq, 1.0 / b, b
+        if (m + 1) % check_every == 0 or m == maxit - 1 or exhausted:
+            w, v = sl.eigh_tridiagonal(np.array(al), np.array(be[:-1]), select="i", select_range=(0, 0))
+            theta, res = float(w[0]), abs(be[-1] * v[-1, 0]) / max(abs(float(w[0])), 1e-300)
+            if progress:
+                progress(m + 1, theta, res, time.perf_counter() - t0)
+            if res < tol or exhausted:
+                break
+    wall = time.perf_counter() - t0
+    del q, p, y
+    return {"E0": theta, "steps": len(al), "relative_residual": res, "wall_s": wall}

@@ -186,7 +186,11 @@ uint64_t unrank_ref(int64_t index, int N, int nup) {
 }
 
 // Gosper's hack as in Basis::generate_int_basis, src/basis.cpp:208-213.
+// The reference never steps past the last state of a sector; the loops here
+// do, once, and the empty state (nup = 0, a sector of one state) has no
+// lowest set bit to divide by, so it maps to itself.
 inline uint64_t gosper_next(uint64_t e) {
+  if (e == 0) return 0;
   uint64_t u = e & (0 - e);
   uint64_t v = u + e;
   return (((v ^ e) / u) >> 2) + v;

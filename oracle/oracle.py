@@ -48,6 +48,9 @@ PARITY = {
     "rd_seeds/rd_hi": "test_tools_random1 literals",
     "sublattices": "test_sublattice1-3 literals",
     "dsf": "dsf_d0_1.ref / dsf_d0_2.ref energies and multiplet weights",
+    "m2_af/m2_str": "closed forms: Neel and polarised states (m2_AF = 1), stripes, random basis states, "
+                    "singlet products inside / across the AF sublattices",
+    "hv_matfree": "equal to the assembled CSR (build_*.ref) times x on 6 models with and without fields",
 }
 
 

@@ -35,3 +35,62 @@ def magnon_levels(lat, N, lx, ly, J1, D1, J2, D2):
 # (lattice, N, Lx, Ly): periodic chains and square lattices up to N = 40
 CLOSED = [("chain", 16, 0, 0), ("chain", 40, 0, 0), ("square", 24, 4, 6), ("square", 36, 6, 6),
           ("square", 40, 4, 10)]
+
+
+# --- sublattice magnetisation of states whose correlators are known exactly --
+# For a product state |s> (one basis state): <S_i.S_j> = 3/4 for i = j
+# (S(S+1) with S = 1/2) and s_i s_j with s = +-1/2 otherwise (the S^+S^-
+# terms move the state off itself).  Summed over the pairs of one sublattice
+# with u up and d down spins: 3n/4 + ((u-d)^2 - n)/4 = n/2 + (u-d)^2/4.
+# For a product of two-site singlets: <S_a.S_b> = -3/4 inside a singlet,
+# 0 between different singlets (each has <S^x,y,z> = 0), 3/4 on the diagonal.
+#
+# Normalisation: m^2_AF is normalised so that the Neel state gives exactly 1
+# (two sublattices of N/2 aligned spins: 2 (N/4 + N^2/16) = N (N+4)/8, whence
+# the 8 / (N (N+4)) of Phys::SquaredSubLattMagAF); m^2_STR is the average of
+# the two stripe orientations' order parameters, each normalised like the AF
+# one, whence 8/2 = 4 over the four striped sublattices.
+
+
+def product_block_sum(bits: int, sl) -> float:
+    """sum_{i,j in sl} <s|S_i.S_j|s> for the basis state s (bit i = spin i up)."""
+    u = sum((bits >> i) & 1 for i in sl)
+    d = len(sl) - u
+    return len(sl) / 2.0 + (u - d) ** 2 / 4.0
+
+
+def singlet_block_sum(pairs, sl) -> float:
+    """sum_{i,j in sl} <S_i.S_j> for the product of singlets on `pairs`."""
+    s = set(sl)
+    acc = 0.75 * len(sl)
+    for a, b in pairs:
+        if a in s and b in s:
+            acc += 2 * (-0.75)
+    return acc
+
+
+def m2_closed(blocks, N: int, orientations: int = 1) -> float:
+    """Sublattice sums normalised by the Neel state's sum N (N+4)/8, averaged
+    over `orientations` (1: AF; 2: the x and y stripes of Striped)."""
+    return sum(blocks) / orientations / (N * (N + 4) / 8.0)
+
+
+def neel_bits(sublattice_a) -> int:
+    return sum(1 << i for i in sublattice_a)
+
+
+def singlet_product(pairs, rank):
+    """Amplitudes of prod_{(a,b)} (|up_a dn_b> - |dn_a up_b>)/sqrt(2) as a
+    dict rank -> amplitude (rank: state -> basis index)."""
+    import itertools
+
+    amp = 2.0 ** (-len(pairs) / 2.0)
+    out = {}
+    for choice in itertools.product((0, 1), repeat=len(pairs)):
+        s, sign = 0, 1.0
+        for (a, b), c in zip(pairs, choice):
+            s |= 1 << (b if c else a)
+            if c:
+                sign = -sign
+        out[rank(s)] = sign * amp
+    return out

@@ -450,6 +450,83 @@ TEST(Phys, SquaredSubLattMagAF_square4x6) {
   EXPECT_EQ(0, VecDestroy(&xr));
 }
 
+/* Colex rank sum_k C(p_k, k) with exact integer binomials (the basis order
+ * of src/basis.cpp:192-215). */
+static PetscInt colex_rank(uint64_t s) {
+  auto C = [](int n, int k) -> PetscInt {
+    if (k < 0 || k > n) return 0;
+    PetscInt c = 1;
+    for (int i = 1; i <= k; ++i) c = c * (n - k + i) / i;
+    return c;
+  };
+  PetscInt r = 0;
+  int k = 0;
+  for (int p = 0; p < 64; ++p)
+    if ((s >> p) & 1) r += C(p, ++k);
+  return r;
+}
+
+/* m^2_AF / m^2_STR (include/phys.tmpl_func.hpp:5-81) on 4x4 states whose
+ * <S_i.S_j> are known exactly: 3/4 on the diagonal; s_i s_j for a basis
+ * state; -3/4 inside a singlet and 0 between different singlets.  The Neel
+ * state gives m^2_AF = 1 (the normalisation); up spins on the even rows give
+ * m^2_STR = (N+8)/(2(N+4)) = 0.6 and m^2_AF = 4/(N+4) = 0.2; singlets inside
+ * one AF sublattice give m^2_AF = 0, singlets across the two 6/(N+4) = 0.3. */
+TEST(Phys, SquaredSubLattMag_closed_forms) {
+  env.nspins = 16;
+  Basis b{env, 0};
+  square2D lat{env, 4, 4};
+  Hamiltonian<square2D> h{env, b, lat, 1.0, 1.0, 0.5, 0.5};
+  ASSERT_EQ(0, h.build_diag(env));
+  ASSERT_EQ(0, h.build_off_diag(env));
+  AF<square2D> af{lat, 2};
+  Striped<square2D> st{lat, 4};
+  Vec x;
+  ASSERT_EQ(0, MatCreateVecs(h.hamilt, NULL, &x));
+  std::vector<PetscScalar> host(size_t(b.size), 0.0);
+  auto load = [&](const std::vector<std::pair<uint64_t, double>>& amps) {
+    std::fill(host.begin(), host.end(), 0.0);
+    for (auto& a : amps) host[size_t(colex_rank(a.first))] = a.second;
+    return VecCopyFromHost(x, host.data());
+  };
+  uint64_t neel = 0;
+  for (PetscInt i : af.get_sl(0)) neel |= 1ull << i;
+  ASSERT_EQ(0, load({{neel, 1.0}}));
+  EXPECT_NEAR(1.0, Phys::SquaredSubLattMagAF(env, b, af, x), 1e-14);
+  uint64_t rows = 0;
+  for (int ix = 0; ix < 4; ++ix)
+    for (int iy = 0; iy < 4; iy += 2) rows |= 1ull << (ix * 4 + iy);
+  ASSERT_EQ(0, load({{rows, 1.0}}));
+  EXPECT_NEAR(0.6, Phys::SquaredSubLattMagSTR(env, b, st, x), 1e-14);
+  EXPECT_NEAR(0.2, Phys::SquaredSubLattMagAF(env, b, af, x), 1e-14);
+  for (int across = 0; across < 2; ++across) {
+    const auto& A = af.get_sl(0);
+    const auto& B = af.get_sl(1);
+    std::vector<std::pair<int, int>> pairs;
+    if (across) {
+      for (size_t k = 0; k < A.size(); ++k) pairs.push_back({int(A[k]), int(B[k])});
+    } else {
+      for (size_t k = 0; k < A.size(); k += 2) pairs.push_back({int(A[k]), int(A[k + 1])});
+      for (size_t k = 0; k < B.size(); k += 2) pairs.push_back({int(B[k]), int(B[k + 1])});
+    }
+    std::vector<std::pair<uint64_t, double>> amps;
+    const double amp = std::pow(2.0, -0.5 * double(pairs.size()));
+    for (uint32_t c = 0; c < (1u << pairs.size()); ++c) {
+      uint64_t s = 0;
+      double sign = 1.0;
+      for (size_t p = 0; p < pairs.size(); ++p) {
+        const bool second = (c >> p) & 1;
+        s |= 1ull << (second ? pairs[p].second : pairs[p].first);
+        if (second) sign = -sign;
+      }
+      amps.push_back({s, sign * amp});
+    }
+    ASSERT_EQ(0, load(amps));
+    EXPECT_NEAR(across ? 0.3 : 0.0, Phys::SquaredSubLattMagAF(env, b, af, x), 1e-13);
+  }
+  EXPECT_EQ(0, VecDestroy(&x));
+}
+
 /* Reads <path>/dsf_d<k>.dat: header lines and, per spin, nev rows of
  * (E_n - E_0, amplitude). */
 struct DsfFile {

@@ -81,6 +81,43 @@ def test_batch_disorder_sample_vs_plain(O, torch):
     assert abs(e0[0] - ref0) <= 1e-9 * abs(ref0)
 
 
+@pytest.mark.timeout(600)
+def test_batch_full_disorder_workload(O, torch):
+    """The whole 256-realisation workload (64 chains of N = 18, 64 of 20, 128
+    of 22; J1 = D1 = 1, J2 = D2 = 0.2, h ~ U[-2, 2] from mt19937_64(1 + k)),
+    solved as one batch to a 1e-9 residual: every E0 equals the plain
+    one-at-a-time Krylov-Schur solve (the loop of examples/main_disorder.cpp:
+    82-120) to 1e-9 relative, and the first and last realisation of each N
+    equal the oracle's sparse eigensolve of the assembled matrix to 1e-9."""
+    from lsquantumed_amd import Solver
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    sizes, counts = [18, 20, 22], [64, 64, 128]
+    hams = disorder_chains(sizes, counts, J1=1.0, J2=0.2, W=2.0, seed=1)
+    say("full batch", sizes, counts)
+    b = BatchSolver(hams, tol=1e-9)
+    e0 = b.solve()
+    assert all(b.info(k)[2] for k in range(len(hams)))
+    say("plain loop")
+    worst = 0.0
+    for k, h in enumerate(hams):
+        eps = Solver.SolverInit(h, 1, tol=1e-9)
+        assert Solver.solve(None, eps) >= 1, k
+        lam, _ = Solver.solution(eps, 0)
+        Solver.SolverClean(eps)
+        worst = max(worst, abs(e0[k] - lam) / abs(lam))
+    say("max rel diff vs plain", worst)
+    assert worst <= 1e-9
+    first = 0
+    for N, c in zip(sizes, counts):
+        for k in (first, first + c - 1):
+            assert hams[k].basis.nspins == N
+            say("oracle", N, k)
+            ref = oracle_ground(O, N, 1.0, 0.2, hams[k].hi)
+            assert abs(e0[k] - ref) <= 1e-9 * abs(ref), (N, k, e0[k], ref)
+        first += c
+    b.destroy()
+
+
 def test_batch_schedules_agree(torch):
     """Every H.v schedule gives the same E0 (the Lanczos recurrence is the
     same up to rounding): the chunk-table kernel at 1, 2 and 4 lanes per

@@ -87,6 +87,25 @@ def test_unrank_rank_large_sector_sample(O, torch, N, lx, ly):
         assert np.array_equal(h.rank(torch.from_numpy(st).cuda()).cpu().numpy(), np.arange(row0, row0 + 4096))
 
 
+@pytest.mark.parametrize("N,mag", [(12, 0), (16, -3), (36, 0)])
+def test_rank_rejects_states_outside_sector(O, torch, N, mag):
+    """lsqed_rank writes -1 for a state with a bit at or above N or with a
+    number of up spins other than the sector's, and the exact rank for the
+    sector states mixed in with them."""
+    say("rank outside the sector", N, mag)
+    h = make("chain", N, mag)
+    nup = N // 2 + mag
+    good = [O.unrank_ref(r, N, nup) for r in (0, 1, h.hamilt_size() // 2, h.hamilt_size() - 1)]
+    top = (1 << N) - 1
+    bad = [good[0] | (1 << N), good[1] ^ 1 if good[1] & 1 else good[1] | 1, 0, top, 1 << 63, (1 << 64) - 1,
+           (good[2] << 1) & top if bin((good[2] << 1) & top).count("1") != nup else 1 << (N + 2)]
+    states = good + bad
+    u64 = np.array(states, dtype=np.uint64)
+    got = h.rank(torch.from_numpy(u64.view(np.int64)).cuda()).cpu().numpy()
+    want = [O.rank_ref(s, N) for s in good] + [-1] * len(bad)
+    assert list(got) == want, (got, want)
+
+
 def test_export_cols_reference_literals(O, torch, literals):
     """Every get_coup_elems literal of the reference's tests (emission order,
     duplicates included), through the device column export."""

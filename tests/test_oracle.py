@@ -12,7 +12,8 @@ import math
 import numpy as np
 import pytest
 
-from closed_forms import CLOSED, magnon_levels, polarised_energy
+from closed_forms import (CLOSED, m2_closed, magnon_levels, neel_bits, polarised_energy, product_block_sum,
+                          singlet_block_sum, singlet_product)
 
 LT = {"chain": 0, "square": 1, "honeycomb": 2}
 
@@ -354,3 +355,103 @@ def test_cpu_lanczos_complex_layout(O):
     y = np.empty_like(x)
     A.matvec(x, y)
     np.testing.assert_allclose(y.real, m.csr(hi, 7) @ x.real, rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------- sublattice magnetisation ----
+# m^2_AF / m^2_STR (Phys::SquaredSubLattMagAF/STR, include/phys.tmpl_func.hpp:
+# 5-81) on states whose <S_i.S_j> are known exactly (tests/closed_forms.py):
+# the Neel state (m^2_AF = 1 by the normalisation), the fully polarised
+# state, stripes, random basis states and products of singlets inside one
+# sublattice (m^2_AF = 0: checks the S^+S^- half of SiSj) or across the two.
+def _mag_cases(O):
+    # (lattice id, N, lx, ly)
+    return [(O.CHAIN, 8, 0, 0), (O.CHAIN, 12, 0, 0), (O.SQUARE, 16, 4, 4), (O.SQUARE, 12, 2, 6),
+            (O.HONEYCOMB, 16, 4, 4)]
+
+
+def _onehot(O, N, bits):
+    nup = bin(bits).count("1")
+    psi = np.zeros(O.compute_size(N, nup))
+    psi[O.rank_ref(bits, N)] = 1.0
+    return nup, psi
+
+
+def test_m2_af_neel_and_polarised(O):
+    for lt, N, lx, ly in _mag_cases(O):
+        af = O.af_sublattices(lt, N, lx, ly)
+        for bits in (neel_bits(af[0]), neel_bits(af[1]), (1 << N) - 1, 0):
+            nup, psi = _onehot(O, N, bits)
+            assert O.m2_af(N, nup, psi, af) == pytest.approx(1.0, abs=1e-14), (lt, N, bits)
+
+
+def test_m2_product_states(O):
+    rng = np.random.default_rng(11)
+    for lt, N, lx, ly in _mag_cases(O):
+        af = O.af_sublattices(lt, N, lx, ly)
+        st = O.striped_sublattices(lx, ly) if lt == O.SQUARE else None
+        for _ in range(6):
+            bits = int(sum(1 << int(i) for i in rng.choice(N, size=N // 2, replace=False)))
+            nup, psi = _onehot(O, N, bits)
+            want = m2_closed([product_block_sum(bits, sl) for sl in af], N)
+            assert O.m2_af(N, nup, psi, af) == pytest.approx(want, abs=1e-14)
+            if st is not None:
+                want = m2_closed([product_block_sum(bits, sl) for sl in st], N, orientations=2)
+                assert O.m2_str(N, nup, psi, st) == pytest.approx(want, abs=1e-14)
+
+
+def test_m2_str_stripes(O):
+    # up spins on the even rows (iy even): one orientation is perfectly
+    # ordered (its two sublattices are aligned), the other averages out, so
+    # m^2_STR = (1 + 4/(N+4)) / 2 = (N+8) / (2 (N+4)); m^2_AF = 4 / (N+4).
+    for lx, ly in ((4, 4), (2, 6), (2, 4)):
+        N = lx * ly
+        bits = sum(1 << (ix * ly + iy) for ix in range(lx) for iy in range(0, ly, 2))
+        nup, psi = _onehot(O, N, bits)
+        st = O.striped_sublattices(lx, ly)
+        af = O.af_sublattices(O.SQUARE, N, lx, ly)
+        assert O.m2_str(N, nup, psi, st) == pytest.approx((N + 8) / (2 * (N + 4)), abs=1e-14)
+        assert O.m2_af(N, nup, psi, af) == pytest.approx(4 / (N + 4), abs=1e-14)
+
+
+def test_m2_singlet_products(O):
+    for lt, N, lx, ly in _mag_cases(O):
+        af = O.af_sublattices(lt, N, lx, ly)
+        a, b = af
+        same = [(a[k], a[k + 1]) for k in range(0, len(a), 2)] + [(b[k], b[k + 1]) for k in range(0, len(b), 2)]
+        cross = list(zip(a, b))
+        for pairs, want in ((same, 0.0), (cross, 6.0 / (N + 4))):
+            amp = singlet_product(pairs, lambda s: O.rank_ref(s, N))
+            psi = np.zeros(O.compute_size(N, N // 2))
+            for r, v in amp.items():
+                psi[r] = v
+            assert m2_closed([singlet_block_sum(pairs, sl) for sl in af], N) == pytest.approx(want, abs=1e-15)
+            assert O.m2_af(N, N // 2, psi, af) == pytest.approx(want, abs=1e-13), (lt, N, pairs)
+            if lt == O.SQUARE:
+                st = O.striped_sublattices(lx, ly)
+                want_s = m2_closed([singlet_block_sum(pairs, sl) for sl in st], N, orientations=2)
+                assert O.m2_str(N, N // 2, psi, st) == pytest.approx(want_s, abs=1e-13)
+
+
+# ------------------------------------------------ matrix-free product ----
+@pytest.mark.parametrize("lt,N,nup,lx,ly,J1,D1,J2,D2", [
+    (0, 12, 6, 0, 0, 1.0, 1.0, 0.0, 0.0),
+    (0, 14, 6, 0, 0, 1.0, 0.3, 0.2, -0.4),
+    (1, 12, 6, 3, 4, 1.0, 1.0, 0.5, 0.5),
+    (1, 16, 9, 4, 4, 0.7, 1.0, -1.0, 0.2),
+    (2, 12, 6, 2, 6, 1.0, 1.0, 0.5, 0.5),
+    (1, 8, 4, 2, 4, 1.0, 1.0, 1.0, 1.0),
+])
+def test_hv_matfree_equals_csr(O, lt, N, nup, lx, ly, J1, D1, J2, D2):
+    """oq_hv_matfree (the oracle's product without an assembled matrix, used
+    for sectors whose CSR does not fit) equals the assembled CSR (pinned by
+    the build_*.ref dumps) times x, with and without fields."""
+    m = O.Model(lt, N, nup, J1, D1, J2, D2, lx=lx, ly=ly)
+    rng = np.random.default_rng(N + nup)
+    x = rng.uniform(-1, 1, m.D)
+    hi = rng.uniform(-1, 1, N)
+    for h, flags in ((None, 3), (hi, 7)):
+        ref = m.csr(h, flags) @ x
+        got = m.hv_matfree(x, hi=h)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+        np.testing.assert_allclose(m.hv_matfree(x, hi=h, row0=7, nrows=20), ref[7:27], rtol=0,
+                                   atol=1e-13 * max(1.0, np.abs(ref).max()))
