@@ -110,6 +110,16 @@ int lsqed_ham_set_couplings(lsqed_ham* h, double J1, double D1, double J2, doubl
  * on the device, must not alias. */
 int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
 
+/* y = H x on complex vectors, as a complex-scalar PETSc build stores them
+ * (README.md:18 of the reference requires complex PETSc; the MatMult of
+ * EPSSolve, src/solver.cpp:45): x and y are D complex128 entries each,
+ * interleaved (re, im) on the device, must not alias.  H is real, so both
+ * parts are gathered together and each part equals lsqed_hv of that part
+ * (same kernel arithmetic).  Uses the specialised row split when that is the
+ * handle's kernel (lsqed_ham_get_hv_kernel == 3), else the chunk-table
+ * kernel. */
+int lsqed_hv_c128(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream);
+
 /* H.v kernel choice: 0 = automatic (specialised row split when N <= 40 and
  * hipRTC is available, else the row split when N <= 32, else the generic
  * chunk-table kernel), 1 = generic chunk-table kernel, 2 = row-split kernel
@@ -126,6 +136,9 @@ int lsqed_ham_get_hv_kernel(lsqed_ham* h);
  * tests/unit_tests/build_diag/chain_1d/test_build_diag60.cpp:29): entries the
  * reference never sets are 0.  d: D doubles (device). */
 int lsqed_diag(lsqed_ham* h, double* d, lsqed_stream_t stream);
+/* The same diagonal into D complex128 entries (imaginary parts 0), for
+ * MATOP_GET_DIAGONAL of a complex-scalar PETSc. */
+int lsqed_diag_c128(lsqed_ham* h, double* d, lsqed_stream_t stream);
 
 /* Column lists of rows [row0, row0+nrows) for one bond kind, in the
  * emission order of HamiltHelper::get_coup_elems (tmpl:350-387,

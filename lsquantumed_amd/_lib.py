@@ -57,6 +57,8 @@ SIGNATURES = {
     "lsqed_ham_get_params": (_i32, [_vp, _P(_dbl), _P(_dbl), _P(_dbl), _P(_dbl), _P(_i32), _P(_i32), _P(_i32)]),
     "lsqed_ham_set_fields": (_i32, [_vp, _vp]),
     "lsqed_hv": (_i32, [_vp, _vp, _vp, _vp]),
+    "lsqed_hv_c128": (_i32, [_vp, _vp, _vp, _vp]),
+    "lsqed_diag_c128": (_i32, [_vp, _vp, _vp]),
     "lsqed_ham_set_hv_kernel": (_i32, [_vp, _i32]),
     "lsqed_ham_get_hv_kernel": (_i32, [_vp]),
     "lsqed_ham_set_couplings": (_i32, [_vp, _dbl, _dbl, _dbl, _dbl]),

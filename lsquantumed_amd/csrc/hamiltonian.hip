@@ -114,7 +114,7 @@ constexpr int kHvThreads = 256;
 constexpr int kBatch = 8;
 
 // --------------------------------------------------------------- H.v kernel --
-template <typename S, typename R, int W, int NC, bool DOT>
+template <typename S, typename R, int W, int NC, bool DOT, bool CPLX = false>
 __global__ __launch_bounds__(kHvThreads) void hv_kernel(RowParams p, const R* __restrict__ gtab,
                                                         const R* __restrict__ gbinom,
                                                         const uint32_t* __restrict__ bonds,
@@ -143,7 +143,8 @@ __global__ __launch_bounds__(kHvThreads) void hv_kernel(RowParams p, const R* __
     int kc[NC];
     R tc[NC];
     chunk_terms<S, R, W, NC>(s, kc, tc, s_tab, p);
-    double acc = 0.0;
+    typedef Entry<CPLX> E;
+    typename E::T acc = E::zero();
     int cnt[2] = {0, 0};
 #pragma unroll
     for (int kind = 0; kind < 2; ++kind) {
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(kHvThreads) void hv_kernel(RowParams p, const R* __
       const double jh = p.jh[kind];
       int b = 0;
       for (; b + kBatch <= nb; b += kBatch) {
-        double vals[kBatch];
+        typename E::T vals[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
           const uint32_t bd = __builtin_amdgcn_readfirstlane(bk[b + u]);
@@ -162,14 +163,14 @@ __global__ __launch_bounds__(kHvThreads) void hv_kernel(RowParams p, const R* __
           const S sm = s & m;
           const bool anti = (sm != 0) && (sm != m);
           cnt[kind] += anti;
-          vals[u] = 0.0;
+          vals[u] = E::zero();
           if (fl && anti) {
             const R rn = flip_rank<S, R, W, NC>((R)row, s, lo, hi, kc, tc, s_tab, p);
-            vals[u] = x[rn];
+            vals[u] = E::load(x, (int64_t)rn);
           }
         }
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) acc += jh * vals[u];
+        for (int u = 0; u < kBatch; ++u) madd(acc, jh, vals[u]);
       }
       for (; b < nb; ++b) {
         const uint32_t bd = __builtin_amdgcn_readfirstlane(bk[b]);
@@ -180,16 +181,16 @@ __global__ __launch_bounds__(kHvThreads) void hv_kernel(RowParams p, const R* __
         cnt[kind] += anti;
         if (fl && anti) {
           const R rn = flip_rank<S, R, W, NC>((R)row, s, lo, hi, kc, tc, s_tab, p);
-          acc += jh * x[rn];
+          madd(acc, jh, E::load(x, (int64_t)rn));
         }
       }
     }
     const int c0 = p.nb[0] - 2 * cnt[0], c1 = p.nb[1] - 2 * cnt[1];
     const double d = diag_value(p, c0, c1, (uint64_t)s, s_f);
-    const double xr = x[row];
-    const double yr = d * xr + acc;
-    y[row] = yr;
-    if (DOT) dotacc += xr * yr;
+    const typename E::T xr = E::load(x, row);
+    const typename E::T yr = diag_apply(d, xr, acc);
+    E::store(y, row, yr);
+    if constexpr (DOT && !CPLX) dotacc += xr * yr;
   }
   if (DOT) {
     double t = block_sum(dotacc, s_red);
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kHvThreads) void hv_kernel(RowParams p, const R* __
 // ------------------------------------------------------- auxiliary kernels --
 template <typename S, typename R, int W, int NC>
 __global__ void diag_kernel(RowParams p, const R* __restrict__ tab, const R* __restrict__ binom,
-                            const uint32_t* __restrict__ bonds, double* __restrict__ d) {
+                            const uint32_t* __restrict__ bonds, double* __restrict__ d, int cplx) {
   for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < p.D;
        row += (int64_t)gridDim.x * blockDim.x) {
     const S s = unrank_row<S, R>((R)row, p.N, p.nup, binom);
@@ -212,7 +213,11 @@ __global__ void diag_kernel(RowParams p, const R* __restrict__ tab, const R* __r
         cnt[kind] += (((s >> lo) ^ (s >> hi)) & 1) ? 1 : 0;
       }
     }
-    d[row] = diag_value(p, p.nb[0] - 2 * cnt[0], p.nb[1] - 2 * cnt[1], (uint64_t)s, p.fields);
+    const double v = diag_value(p, p.nb[0] - 2 * cnt[0], p.nb[1] - 2 * cnt[1], (uint64_t)s, p.fields);
+    if (cplx)
+      reinterpret_cast<double2*>(d)[row] = make_double2(v, 0.0);
+    else
+      d[row] = v;
   }
 }
 
@@ -286,12 +291,12 @@ int hv_grid(const lsqed_ham* h, int64_t ntiles) {
   return (int)g;
 }
 
-template <typename S, typename R, int W, int NC, bool DOT>
+template <typename S, typename R, int W, int NC, bool DOT, bool CPLX = false>
 static int launch_hv_t(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t st) {
   RowParams p = make_row_params(h);
   const int64_t ntiles = (h->D + kHvThreads - 1) / kHvThreads;
   const size_t sm = hv_smem_bytes(h);
-  auto kfn = hv_kernel<S, R, W, NC, DOT>;
+  auto kfn = hv_kernel<S, R, W, NC, DOT, CPLX>;
   if (sm > 64 * 1024) LSQED_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   hipLaunchKernelGGL(kfn, dim3(grid), dim3(kHvThreads), sm, st, p, (const R*)h->d_table, (const R*)h->d_binom,
                      (const uint32_t*)h->d_bonds, x, y, dot_part, ntiles);
@@ -338,6 +343,21 @@ int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, hi
   }
   if (dot_partials) return launch_hv_t<uint64_t, uint64_t, 6, kMaxChunks, true>(h, x, y, dot_partials, grid, s);
   return launch_hv_t<uint64_t, uint64_t, 6, kMaxChunks, false>(h, x, y, nullptr, grid, s);
+}
+
+// y = H x on interleaved complex vectors (D complex128 entries each): the
+// specialised row split when that is the handle's kernel, else the generic
+// chunk-table kernel (the unspecialised row split has no complex form).
+int launch_hv_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s) {
+  if (h->D == 0) return LSQED_OK;
+  if (hv_kind(h) == 3) {
+    int rc = split_plan_create(h);
+    if (rc) return rc;
+    return launch_hv_jit_c128(h, x, y, s);
+  }
+  const int grid = hv_grid(h, (h->D + kHvThreads - 1) / kHvThreads);
+  if (!h->tab.wide) return launch_hv_t<uint32_t, uint32_t, 8, 4, false, true>(h, x, y, nullptr, grid, s);
+  return launch_hv_t<uint64_t, uint64_t, 6, kMaxChunks, false, true>(h, x, y, nullptr, grid, s);
 }
 
 static int aux_grid(int64_t n) {
@@ -480,7 +500,7 @@ int lsqed_ham_set_build(lsqed_ham* h, int flags) {
     return fail(LSQED_ERR_ARG, "unknown build flag");
   if ((flags & LSQED_BUILD_DISORDER) && !h->d_fields)
     return fail(LSQED_ERR_STATE, "build_disorder needs fields (lsqed_ham_set_fields)");
-  if (flags != h->build) h->jit = nullptr;  // the specialised kernel bakes in the build flags
+  if (flags != h->build) h->jit = h->jit_c = nullptr;  // the specialised kernel bakes in the build flags
   h->build = flags;
   return LSQED_OK;
 }
@@ -489,14 +509,14 @@ int lsqed_ham_set_fields(lsqed_ham* h, const double* hi) {
   if (!h) return fail(LSQED_ERR_ARG, "null handle");
   if (!hi) {
     h->fields.clear();
-    if (h->build & LSQED_BUILD_DISORDER) h->jit = nullptr;
+    if (h->build & LSQED_BUILD_DISORDER) h->jit = h->jit_c = nullptr;
     h->build &= ~LSQED_BUILD_DISORDER;
     return LSQED_OK;
   }
   h->fields.assign(hi, hi + h->N);
   if (!h->d_fields) LSQED_HIP(hipMalloc((void**)&h->d_fields, h->N * sizeof(double)));
   LSQED_HIP(hipMemcpy(h->d_fields, hi, h->N * sizeof(double), hipMemcpyHostToDevice));
-  if (!(h->build & LSQED_BUILD_DISORDER)) h->jit = nullptr;
+  if (!(h->build & LSQED_BUILD_DISORDER)) h->jit = h->jit_c = nullptr;
   h->build |= LSQED_BUILD_DISORDER;
   return LSQED_OK;
 }
@@ -509,7 +529,7 @@ int lsqed_ham_set_couplings(lsqed_ham* h, double J1, double D1, double J2, doubl
   h->Dz[0] = D1;
   h->J[1] = J2;
   h->Dz[1] = D2;
-  h->jit = nullptr;  // the specialised kernel bakes in the couplings
+  h->jit = h->jit_c = nullptr;  // the specialised kernel bakes in the couplings
   return LSQED_OK;
 }
 
@@ -517,6 +537,12 @@ int lsqed_hv(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream) {
   if (!h || !x || !y) return fail(LSQED_ERR_ARG, "null argument");
   if (x == y) return fail(LSQED_ERR_ARG, "x and y must not alias");
   return launch_hv(h, x, y, nullptr, (hipStream_t)stream);
+}
+
+int lsqed_hv_c128(lsqed_ham* h, const double* x, double* y, lsqed_stream_t stream) {
+  if (!h || !x || !y) return fail(LSQED_ERR_ARG, "null argument");
+  if (x == y) return fail(LSQED_ERR_ARG, "x and y must not alias");
+  return launch_hv_c128(h, x, y, (hipStream_t)stream);
 }
 
 int lsqed_ham_get_hv_kernel(lsqed_ham* h) { return h ? hv_kind(h) : 0; }
@@ -536,20 +562,25 @@ int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant) {
   return LSQED_OK;
 }
 
-int lsqed_diag(lsqed_ham* h, double* d, lsqed_stream_t stream) {
+static int diag_launch(lsqed_ham* h, double* d, int cplx, hipStream_t stream) {
   if (!h || !d) return fail(LSQED_ERR_ARG, "null argument");
   if (h->D == 0) return LSQED_OK;
   RowParams p = make_row_params(h);
   const int g = aux_grid(h->D);
   if (!h->tab.wide)
-    hipLaunchKernelGGL((diag_kernel<uint32_t, uint32_t, 8, 4>), dim3(g), dim3(256), 0, (hipStream_t)stream, p,
-                       (const uint32_t*)h->d_table, (const uint32_t*)h->d_binom, h->d_bonds, d);
+    hipLaunchKernelGGL((diag_kernel<uint32_t, uint32_t, 8, 4>), dim3(g), dim3(256), 0, stream, p,
+                       (const uint32_t*)h->d_table, (const uint32_t*)h->d_binom, h->d_bonds, d, cplx);
   else
-    hipLaunchKernelGGL((diag_kernel<uint64_t, uint64_t, 6, kMaxChunks>), dim3(g), dim3(256), 0,
-                       (hipStream_t)stream, p, (const uint64_t*)h->d_table, (const uint64_t*)h->d_binom,
-                       h->d_bonds, d);
+    hipLaunchKernelGGL((diag_kernel<uint64_t, uint64_t, 6, kMaxChunks>), dim3(g), dim3(256), 0, stream, p,
+                       (const uint64_t*)h->d_table, (const uint64_t*)h->d_binom, h->d_bonds, d, cplx);
   LSQED_HIP(hipGetLastError());
   return LSQED_OK;
+}
+
+int lsqed_diag(lsqed_ham* h, double* d, lsqed_stream_t stream) { return diag_launch(h, d, 0, (hipStream_t)stream); }
+
+int lsqed_diag_c128(lsqed_ham* h, double* d, lsqed_stream_t stream) {
+  return diag_launch(h, d, 1, (hipStream_t)stream);
 }
 
 int lsqed_export_cols(lsqed_ham* h, int kind, int64_t row0, int64_t nrows, int max_cols, int64_t* cols,

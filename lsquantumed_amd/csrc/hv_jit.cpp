@@ -56,6 +56,29 @@ typedef unsigned long long BT;  // row bases and ranks (D >= 2^32)
 typedef unsigned int BT;
 #endif
 
+// Vector entries: FP64, or interleaved complex FP64 (CPLX: a complex-scalar
+// PETSc Vec); H is real, so both parts take the same coefficients, each
+// updated with the expression of the real kernel.
+#ifndef CPLX
+#define CPLX 0
+#endif
+#if CPLX
+typedef double2 XV;
+__device__ __forceinline__ XV xzero() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ void xmadd(XV& a, double c, XV v) {
+  a.x += c * v.x;
+  a.y += c * v.y;
+}
+__device__ __forceinline__ XV xdiag(double d, XV x, XV acc) { return make_double2(d * x.x + acc.x, d * x.y + acc.y); }
+__device__ __forceinline__ double xdot(XV a, XV b) { return a.x * b.x + a.y * b.y; }
+#else
+typedef double XV;
+__device__ __forceinline__ XV xzero() { return 0.0; }
+__device__ __forceinline__ void xmadd(XV& a, double c, XV v) { a += c * v; }
+__device__ __forceinline__ XV xdiag(double d, XV x, XV acc) { return d * x + acc; }
+__device__ __forceinline__ double xdot(XV a, XV b) { return a * b; }
+#endif
+
 __device__ __forceinline__ BT rfl(BT v) {
 #if WIDE
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
@@ -110,7 +133,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   __shared__ double s_dish[NBUF];
   __shared__ double s_red[kThreads / 64];
 #if ROWSTAGE
-  __shared__ double s_x[RMAX];
+  __shared__ XV s_x[RMAX];
 #endif
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += kThreads) s_RL0[i] = a.RL0[i];
@@ -151,8 +174,8 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     const int st = (int)(t - (long long)pl * a.ntasks);
     const int prob = a.gidx[pl];
     if (!a.active[prob]) continue;  // block-uniform
-    const double* __restrict__ xv = a.x + a.poff[prob];
-    double* __restrict__ yv = a.y + a.poff[prob];
+    const XV* __restrict__ xv = (const XV*)a.x + a.poff[prob];
+    XV* __restrict__ yv = (XV*)a.y + a.poff[prob];
     const double jh_[2] = {a.pcoup[4 * prob], a.pcoup[4 * prob + 1]};
     const double dq0_ = a.pcoup[4 * prob + 2], dq1_ = a.pcoup[4 * prob + 3];
     if (DIAG_MODE == 1 && prob != loaded) {
@@ -181,8 +204,8 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   double dotacc = 0.0;
   int pb = 0;
   for (int t = (int)(blockIdx.x >> 3); t < seg; t += P, pb ^= 1) {
-    const double* __restrict__ xv = a.x;
-    double* __restrict__ yv = a.y;
+    const XV* __restrict__ xv = (const XV*)a.x;
+    XV* __restrict__ yv = (XV*)a.y;
     constexpr double jh_[2] = {JH0, JH1};
     constexpr double dq0_ = DQ0, dq1_ = DQ1;
     const unsigned int task = a.tasks[xcd * seg + t];
@@ -192,8 +215,8 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   double dotacc = 0.0;
   constexpr int pb = 0;
   for (long long t = ta; t < tb; ++t) {
-    const double* __restrict__ xv = a.x;
-    double* __restrict__ yv = a.y;
+    const XV* __restrict__ xv = (const XV*)a.x;
+    XV* __restrict__ yv = (XV*)a.y;
     constexpr double jh_[2] = {JH0, JH1};
     constexpr double dq0_ = DQ0, dq1_ = DQ1;
     const unsigned int task = a.tasks[t];
@@ -247,11 +270,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         for (int q = 0; q < NH; ++q) dh = ((h >> q) & 1u) ? dh + s_f[L + q] : dh - s_f[L + q];
       s_dish[pb] = dh;
     }
-    double stage[kStage];
+    XV stage[kStage];
 #pragma unroll
     for (int u = 0; u < kStage; ++u) {
       const int e = tid + u * kThreads < R ? tid + u * kThreads : R - 1;
-      stage[u] = 0.0;
+      stage[u] = xzero();
       if (u * kThreads < R) stage[u] = xv[B + (BT)e];
     }
     __syncthreads();  // every wave is done with the previous row's x
@@ -310,18 +333,18 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       if (i >= R) break;
 #endif
       const BT r = B + (BT)i;
-      double acc = 0.0;
+      XV acc = xzero();
       // high bonds: row h ^ m, same entry; loads issued in batches of 8
       for (int b0 = 0; b0 < nact; b0 += kG) {
-        double v[kG];
+        XV v[kG];
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
-          v[u] = 0.0;
+          v[u] = xzero();
           if (b0 + u < nact) v[u] = xv[rfl(s_hbase[pb][b0 + u]) + (BT)i];
         }
 #pragma unroll
         for (int u = 0; u < kG; ++u)
-          if (b0 + u < nact) acc += s_hcoef[pb][b0 + u] * v[u];
+          if (b0 + u < nact) xmadd(acc, s_hcoef[pb][b0 + u], v[u]);
       }
       const unsigned int l = Uk[i];
       const unsigned int v0 = l & 255u;
@@ -334,7 +357,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       // gathers of a batch are all in flight together
 #pragma unroll
       for (int b0 = 0; b0 < NCROSS; b0 += kG) {
-        double v[kG];
+        XV v[kG];
         unsigned int onm = 0;
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
@@ -354,7 +377,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
               i2 = t0 + s_RL1[k0 * W1 + (l2 >> 8)];
             }
 #if MASKED_GATHER
-            v[u] = 0.0;
+            v[u] = xzero();
             if (on) v[u] = xv[cb[hp] + (BT)i2];
 #else
             const BT idx = on ? cb[hp] + (BT)i2 : r;
@@ -365,14 +388,14 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
           const int b = b0 + u;
-          if (b < NCROSS) acc += (((onm >> u) & 1u) ? jh_[CRK[b]] : 0.0) * v[u];
+          if (b < NCROSS) xmadd(acc, ((onm >> u) & 1u) ? jh_[CRK[b]] : 0.0, v[u]);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
       // low bonds: inside row h, entry RL(l ^ m)
 #pragma unroll
       for (int b0 = 0; b0 < NLOW; b0 += kG) {
-        double v[kG];
+        XV v[kG];
         unsigned int onm = 0;
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
@@ -397,7 +420,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #if ROWSTAGE
             v[u] = s_x[on ? i2 : (unsigned int)i];
 #elif MASKED_GATHER
-            v[u] = 0.0;
+            v[u] = xzero();
             if (on) v[u] = xv[B + (BT)i2];
 #else
             const BT idx = on ? B + (BT)i2 : r;
@@ -408,7 +431,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
           const int b = b0 + u;
-          if (b < NLOW) acc += (((onm >> u) & 1u) ? jh_[LOWK[b]] : 0.0) * v[u];
+          if (b < NLOW) xmadd(acc, ((onm >> u) & 1u) ? jh_[LOWK[b]] : 0.0, v[u]);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -426,13 +449,13 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         d = (dq0_ * (double)c0 + dq1_ * (double)c1) + 0.5 * dis;
       }
 #if ROWSTAGE
-      const double xr = s_x[i];
+      const XV xr = s_x[i];
 #else
-      const double xr = xv[r];
+      const XV xr = xv[r];
 #endif
-      const double yr = d * xr + acc;
+      const XV yr = xdiag(d, xr, acc);
       yv[r] = yr;
-      if (DOT) dotacc += xr * yr;
+      if (DOT) dotacc += xdot(xr, yr);
     }
 #if BATCH
     if (DOT) {
@@ -565,7 +588,7 @@ static std::string dbl(double v) {
 // Source of the kernel specialised to the bond classes and couplings of `h`
 // (row split with L = split_low_bits(h, batch) low bits, see hv_split.hip).  Fields stay a
 // run-time array, so disorder realisations share one compiled kernel.
-std::string jit_source(const lsqed_ham* h, bool batch) {
+std::string jit_source(const lsqed_ham* h, bool batch, bool cplx) {
   const int N = h->N, nup = h->nup, L = split_low_bits(h, batch), NH = N - L;
   std::vector<unsigned> lowm, him;
   std::vector<int> lowk, lowt, crq, crp, crk, hik;
@@ -590,9 +613,15 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   std::ostringstream o;
   o << "// generated by liblsqed (hv_jit.cpp)\n#define BATCH " << (batch ? 1 : 0) << "\n#define WIDE "
     << (N > 32 ? 1 : 0) << "\n";
+  // Complex entries (16 B): the staged row doubles in LDS (two blocks per CU
+  // at L = 14), so the kernel asks for two waves per SIMD and its registers.
+  if (cplx) o << "#define CPLX 1\n";
   // occupancy hint of the kernel (waves per SIMD); LSQED_JIT_WAVES_PER_EU overrides the default 4
-  if (const char* w = getenv("LSQED_JIT_WAVES_PER_EU"))
+  if (const char* w = getenv("LSQED_JIT_WAVES_PER_EU")) {
     if (atoi(w) >= 1 && atoi(w) <= 8) o << "#define WAVES_PER_EU " << atoi(w) << "\n";
+  } else if (cplx) {
+    o << "#define WAVES_PER_EU 2\n";
+  }
   // Gathers issued together: 16 for single-model kernels with 32-bit ranks
   // and at least ~55 global gathers per entry (cross bonds plus about half
   // the high bonds), else 8.  Results are bit-identical (the sums run in bond
@@ -601,7 +630,7 @@ std::string jit_source(const lsqed_ham* h, bool batch) {
   // chain N=26 0.199 / 0.227; 6x6 (64-bit ranks) 686 / 691; the batched
   // disorder solve 839 / 810 realisations/s.  LSQED_JIT_GATHER_BATCH
   // (2..16) overrides.
-  int gather_batch = (!batch && N <= 32 && 2 * crq.size() + him.size() >= 110) ? 16 : 8;
+  int gather_batch = (!batch && !cplx && N <= 32 && 2 * crq.size() + him.size() >= 110) ? 16 : 8;
   if (const char* w = getenv("LSQED_JIT_GATHER_BATCH"))
     if (atoi(w) >= 2 && atoi(w) <= 16) gather_batch = atoi(w);
   o << "#define GATHER_BATCH " << gather_batch << "\n";
@@ -684,7 +713,7 @@ bool jit_enabled() {
 
 int jit_prepare(lsqed_ham* h) {
   if (h->jit) return LSQED_OK;
-  const std::string src = jit_source(h, false);
+  const std::string src = jit_source(h, false, false);
   std::shared_ptr<JitModule> m;
   {
     std::lock_guard<std::mutex> lk(g_jit_mu);
@@ -705,11 +734,10 @@ int jit_prepare(lsqed_ham* h) {
 // Blocks of one specialised H.v (one per work item, or a persistent block per
 // CU when the kernel stages rows); also the number of x.y partial sums it
 // writes.
-int jit_grid(lsqed_ham* h) {
-  if (jit_prepare(h) != LSQED_OK) return 0;
+static int jit_grid_of(lsqed_ham* h, const JitModule* m) {
   SplitTables t;
   if (split_tables(h, t) != LSQED_OK) return 0;
-  if (((JitModule*)h->jit)->rowstage) {
+  if (m->rowstage) {
     // two rows per block when there are at least 2^15 rows, else one
     // (LSQED_JIT_ROWS_PER_BLOCK overrides): a block's once-per-block work --
     // its tables, and the stores of the 12 VGPRs the kernel spills at four
@@ -726,6 +754,59 @@ int jit_grid(lsqed_ham* h) {
     return 8 * std::max(1, (per_xcd + k - 1) / k);
   }
   return std::max(8, ((t.ntasks_ord + 7) / 8) * 8);
+}
+
+int jit_grid(lsqed_ham* h) {
+  if (jit_prepare(h) != LSQED_OK) return 0;
+  return jit_grid_of(h, (const JitModule*)h->jit);
+}
+
+// The complex128 module of the handle (same bond structure and couplings).
+static int jit_prepare_c128(lsqed_ham* h) {
+  if (h->jit_c) return LSQED_OK;
+  const std::string src = jit_source(h, false, true);
+  std::shared_ptr<JitModule> m;
+  {
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    const auto keyd = std::make_pair(h->device, src);
+    auto it = g_jit_cache.find(keyd);
+    if (it != g_jit_cache.end()) {
+      m = it->second;
+    } else {
+      m = jit_compile(src, false);
+      g_jit_cache[keyd] = m;
+    }
+  }
+  if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the complex specialised H.v failed: " + m->log);
+  h->jit_c = m.get();
+  return LSQED_OK;
+}
+
+int launch_hv_jit_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s) {
+  int rc = jit_prepare_c128(h);
+  if (rc) return rc;
+  SplitTables t;
+  if ((rc = split_tables(h, t))) return rc;
+  JitModule* m = (JitModule*)h->jit_c;
+  const int grid = jit_grid_of(h, m);
+  if (grid <= 0) return fail(LSQED_ERR_HIP, "row-split plan of the complex H.v failed");
+  JitArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.x = x;
+  a.y = y;
+  a.dot = nullptr;
+  a.fields = h->d_fields;
+  a.U = t.U;
+  a.RL0 = t.RL0;
+  a.RL1 = t.RL1;
+  a.binom = t.binom;
+  a.binom64 = t.binom64;
+  a.tasks = m->rowstage ? t.rows_ord : t.tasks_ord;
+  a.ntasks = m->rowstage ? t.nrows_ord : t.ntasks_ord;
+  for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
+  void* args[] = {&a};
+  LSQED_HIP(hipModuleLaunchKernel(m->fn_nodot, (unsigned)grid, 1, 1, m->threads, 1, 1, 0, s, args, nullptr));
+  return LSQED_OK;
 }
 
 int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s) {
@@ -758,7 +839,7 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
 // fields per problem at run time).  *key receives the generated source, which
 // identifies the group of problems that can share the kernel.
 int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key) {
-  const std::string src = jit_source(h, true);
+  const std::string src = jit_source(h, true, false);
   if (key) *key = src;
   std::shared_ptr<JitModule> m;
   {

@@ -79,6 +79,7 @@ struct lsqed_ham {
   void* split = nullptr;       // row-split plan of the single-model kernels (hv_split.hip), built on first use
   void* split_b = nullptr;     // row-split plan of the batched kernel (its own row split)
   void* jit = nullptr;         // specialised row-split module (hv_jit.cpp), cached per bond structure
+  void* jit_c = nullptr;       // its complex128 form (lsqed_hv_c128)
   bool jit_failed = false;
 };
 
@@ -153,6 +154,8 @@ bool split_jit_fits(const lsqed_ham* h, bool batch = false);
 bool jit_enabled();
 int jit_prepare(lsqed_ham* h);
 int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
+int launch_hv_jit_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s);
+int launch_hv_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s);
 int jit_grid(lsqed_ham* h);
 
 // Batch variant of the specialised kernel (hv_jit.cpp): problems sharing a

@@ -68,6 +68,38 @@ __device__ __forceinline__ double diag_value(const RowParams& p, int c0, int c1,
   return d;
 }
 
+// Vector entries: FP64 (real scalars) or interleaved complex FP64 (re, im),
+// as a complex-scalar PETSc Vec stores them.  H is real, so both parts see
+// the same coefficients; each part is updated with the same expression as
+// the real product, so it rounds the same way.
+template <bool CPLX>
+struct Entry {
+  typedef double T;
+  static __device__ __forceinline__ T zero() { return 0.0; }
+  static __device__ __forceinline__ T load(const double* __restrict__ x, int64_t i) { return x[i]; }
+  static __device__ __forceinline__ void store(double* __restrict__ y, int64_t i, T v) { y[i] = v; }
+};
+template <>
+struct Entry<true> {
+  typedef double2 T;
+  static __device__ __forceinline__ T zero() { return make_double2(0.0, 0.0); }
+  static __device__ __forceinline__ T load(const double* __restrict__ x, int64_t i) {
+    return reinterpret_cast<const double2*>(x)[i];
+  }
+  static __device__ __forceinline__ void store(double* __restrict__ y, int64_t i, T v) {
+    reinterpret_cast<double2*>(y)[i] = v;
+  }
+};
+__device__ __forceinline__ void madd(double& acc, double c, double v) { acc += c * v; }
+__device__ __forceinline__ void madd(double2& acc, double c, double2 v) {
+  acc.x += c * v.x;
+  acc.y += c * v.y;
+}
+__device__ __forceinline__ double diag_apply(double d, double x, double acc) { return d * x + acc; }
+__device__ __forceinline__ double2 diag_apply(double d, double2 x, double2 acc) {
+  return make_double2(d * x.x + acc.x, d * x.y + acc.y);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -276,9 +276,25 @@ class Hamiltonian:
         check(lib().lsqed_hv(self.hamilt, ptr(x), ptr(y), stream_ptr(stream)))
         return y
 
-    def diagonal(self, stream=None):
+    def matvec_c128(self, x, y=None, stream=None):
+        """y = H x on complex vectors (complex128 CUDA tensors of length D,
+        the layout of a complex-scalar PETSc Vec); both parts in one pass."""
         import torch
 
+        _check_vec(x, self.size, torch.complex128)
+        if y is None:
+            y = torch.empty_like(x)
+        _check_vec(y, self.size, torch.complex128)
+        check(lib().lsqed_hv_c128(self.hamilt, ptr(x), ptr(y), stream_ptr(stream)))
+        return y
+
+    def diagonal(self, stream=None, complex128=False):
+        import torch
+
+        if complex128:
+            d = torch.empty(self.size, dtype=torch.complex128, device="cuda")
+            check(lib().lsqed_diag_c128(self.hamilt, ptr(d), stream_ptr(stream)))
+            return d
         d = torch.empty(self.size, dtype=torch.float64, device="cuda")
         check(lib().lsqed_diag(self.hamilt, ptr(d), stream_ptr(stream)))
         return d
@@ -312,9 +328,10 @@ class Hamiltonian:
         return out[: states.numel()]
 
 
-def _check_vec(t, n):
+def _check_vec(t, n, dtype=None):
     import torch
 
-    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    dtype = dtype or torch.float64
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()
             and t.numel() == n):
-        raise ValueError(f"expected a contiguous float64 CUDA tensor of {n} elements")
+        raise ValueError(f"expected a contiguous {dtype} CUDA tensor of {n} elements")

@@ -13,7 +13,7 @@
 #include "lsqed_internal.hpp"
 
 namespace lsqed {
-std::string jit_source(const lsqed_ham* h, bool batch);
+std::string jit_source(const lsqed_ham* h, bool batch, bool cplx);
 }
 
 static int failures = 0;
@@ -48,11 +48,13 @@ static int generate(int lat, int N, int lx, int ly) {
     h.Dz[1] = 0.3;
     EXPECT(bonds(lat, N, lx, ly, h));
     if (!lsqed::split_plan_supported(&h)) continue;
-    const std::string s = lsqed::jit_source(&h, false);
+    const std::string s = lsqed::jit_source(&h, false, false);
     EXPECT(s.find("hv_jit_nodot") != std::string::npos);
+    const std::string c = lsqed::jit_source(&h, false, true);
+    EXPECT(c.find("#define CPLX 1") != std::string::npos && c.find("hv_jit_nodot") != std::string::npos);
     ++n;
     if (N <= 32) {
-      const std::string b = lsqed::jit_source(&h, true);
+      const std::string b = lsqed::jit_source(&h, true, false);
       EXPECT(b.find("hv_bjit_nodot") != std::string::npos);
       ++n;
     }
