@@ -211,6 +211,22 @@ int lsqed_lanczos_ground(lsqed_ham* h, int maxit, double tol, uint64_t seed, dou
 int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta, double* y,
                        double* alpha, double* beta_next, lsqed_stream_t stream);
 
+/* lsqed_lanczos_step without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as lsqed_lanczos_step; the partial sums go to a static
+ * device array of 2^22 doubles (32 MiB per device, held until the process
+ * ends), found once per device with hipGetSymbolAddress.  It calls no
+ * hipMalloc/hipFree, so the model's H.v plan must already exist: call
+ * lsqed_hv or lsqed_lanczos_step on the handle once first (and again after
+ * set_fields / set_couplings / set_build / set_hv_kernel), else
+ * LSQED_ERR_STATE.  A sector whose step needs more than 2^22 partial sums
+ * (H.v blocks + reduction blocks + 8; N >= 38 with the specialised kernel) is
+ * rejected with LSQED_ERR_ARG.  Every call on one device shares the array:
+ * calls that overlap on one device (several host threads or streams) race. */
+int lsqed_lanczos_step_noalloc(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta,
+                               double* y, double* alpha, double* beta_next, lsqed_stream_t stream);
+
 /* ----------------------------------------------------------- observables -- */
 /* All pairs <S^z_i S^z_j> (SzSz::ExpectVal, src/spinOperators/correlation.cpp
  * :86-139), <S^+_i S^-_j> (SpSm, :154-224) for a real state psi (device,

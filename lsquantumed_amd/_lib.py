@@ -73,6 +73,7 @@ SIGNATURES = {
     "lsqed_eps_destroy": (_i32, [_vp]),
     "lsqed_lanczos_ground": (_i32, [_vp, _i32, _dbl, _u64, _vp, _P(_dbl), _P(_i32), _vp]),
     "lsqed_lanczos_step": (_i32, [_vp, _vp, _dbl, _vp, _dbl, _dbl, _vp, _P(_dbl), _P(_dbl), _vp]),
+    "lsqed_lanczos_step_noalloc": (_i32, [_vp, _vp, _dbl, _vp, _dbl, _dbl, _vp, _P(_dbl), _P(_dbl), _vp]),
     "lsqed_obs_szsz": (_i32, [_vp, _vp, _vp, _vp]),
     "lsqed_obs_spsm": (_i32, [_vp, _vp, _vp, _vp]),
     "lsqed_obs_sz": (_i32, [_vp, _vp, _vp, _vp]),

@@ -315,6 +315,21 @@ static int hv_kind(lsqed_ham* h) {
   return split_supported(h) ? 2 : 1;
 }
 
+// Whether launch_hv can run without building anything (no hipRTC compile,
+// no row-split tables): the kernel is resolved and its plan exists.
+bool hv_plan_ready(const lsqed_ham* h) {
+  int k = h->hv_variant;
+  if (k == 0) {
+    if (jit_enabled() && !h->jit_failed && split_jit_fits(h))
+      k = h->jit ? 3 : -1;
+    else
+      k = split_supported(h) ? 2 : 1;
+  }
+  if (k == 3) return h->jit && h->split;
+  if (k == 2) return h->split != nullptr;
+  return k == 1;
+}
+
 int hv_launch_grid(lsqed_ham* h) {
   const int k = hv_kind(h);
   if (k >= 2) {

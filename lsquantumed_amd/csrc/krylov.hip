@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lsqed_internal.hpp"
@@ -1059,19 +1060,20 @@ int lsqed_stream_probe(int kind, double* a, const double* b, const double* c, in
   return LSQED_OK;
 }
 
-int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta, double* y,
-                       double* alpha, double* beta_next, lsqed_stream_t stream) {
-  if (!h || !q || !p || !y) return fail(LSQED_ERR_ARG, "null argument");
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace lsqed {
+
+// One three-term Lanczos step (lsqed_lanczos_step) with its partial sums in
+// `scratch` (G + R + 8 doubles: G = hv_launch_grid(h), R = red_blocks(D)).
+static int lanczos_step_core(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta, double* y,
+                             double* alpha, double* beta_next, double* scratch, int G, hipStream_t s) {
   const int64_t D = h->D;
-  const int G = hv_launch_grid(h);
-  if (G <= 0) return fail(LSQED_ERR_HIP, "H.v plan setup failed");
   const int R = red_blocks(D);
-  int rc = ensure_scratch(h, (size_t)G + R + 8);
-  if (rc) return rc;
-  double* dpart = h->d_scratch;
-  double* upart = h->d_scratch + G;
-  double* outv = h->d_scratch + G + R;
+  int rc = LSQED_OK;
+  double* dpart = scratch;
+  double* upart = scratch + G;
+  double* outv = scratch + G + R;
   if ((rc = launch_hv(h, q, y, dpart, s))) return rc;
   double qy = 0.0;
   if ((rc = finish_reduce(h, dpart, G, 1, outv, &qy, s))) return rc;
@@ -1083,6 +1085,66 @@ int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, doub
   if (alpha) *alpha = a;
   if (beta_next) *beta_next = std::sqrt(ss);
   return LSQED_OK;
+}
+
+// Device scratch of lsqed_lanczos_step_noalloc: 2^22 doubles (32 MiB) per
+// device, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host the name is a stand-in, not a device
+// address): noalloc_scratch() resolves the device copy once per device with
+// hipGetSymbolAddress and every use goes through that pointer.
+constexpr size_t kNoAllocDoubles = (size_t)1 << 22;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static double* g_noalloc_ptr[kMaxDevices] = {};
+
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  LSQED_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LSQED_ERR_ARG, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    LSQED_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(LSQED_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_noalloc_ptr[dev] = (double*)ptr;
+  }
+  *out = g_noalloc_ptr[dev];
+  return LSQED_OK;
+}
+
+}  // namespace lsqed
+
+extern "C" {
+
+int lsqed_lanczos_step(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta, double* y,
+                       double* alpha, double* beta_next, lsqed_stream_t stream) {
+  if (!h || !q || !p || !y) return fail(LSQED_ERR_ARG, "null argument");
+  const int G = hv_launch_grid(h);
+  if (G <= 0) return fail(LSQED_ERR_HIP, "H.v plan setup failed");
+  int rc = ensure_scratch(h, (size_t)G + red_blocks(h->D) + 8);
+  if (rc) return rc;
+  return lanczos_step_core(h, q, sq, p, sp, beta, y, alpha, beta_next, h->d_scratch, G, (hipStream_t)stream);
+}
+
+int lsqed_lanczos_step_noalloc(lsqed_ham* h, const double* q, double sq, double* p, double sp, double beta,
+                               double* y, double* alpha, double* beta_next, lsqed_stream_t stream) {
+  if (!h || !q || !p || !y) return fail(LSQED_ERR_ARG, "null argument");
+  if (!hv_plan_ready(h))
+    return fail(LSQED_ERR_STATE,
+                "the H.v plan of this model is not built: call lsqed_hv or lsqed_lanczos_step on it once first "
+                "(and again after lsqed_ham_set_fields / set_couplings / set_build / set_hv_kernel)");
+  const int G = hv_launch_grid(h);
+  if (G <= 0) return fail(LSQED_ERR_HIP, "H.v plan setup failed");
+  const size_t need = (size_t)G + red_blocks(h->D) + 8;
+  if (need > kNoAllocDoubles)
+    return fail(LSQED_ERR_ARG, "lsqed_lanczos_step_noalloc: this sector needs " + std::to_string(need) +
+                                   " doubles of scratch, more than the " + std::to_string(kNoAllocDoubles) +
+                                   " of the static array; use lsqed_lanczos_step");
+  double* sbuf = nullptr;
+  int rc = noalloc_scratch(&sbuf);
+  if (rc) return rc;
+  return lanczos_step_core(h, q, sq, p, sp, beta, y, alpha, beta_next, sbuf, G, (hipStream_t)stream);
 }
 
 int lsqed_lanczos_ground(lsqed_ham* h, int maxit, double tol, uint64_t seed, double* work, double* e0, int* steps,

@@ -122,6 +122,7 @@ int ensure_scratch(lsqed_ham* h, size_t doubles);
 // hv_launch_grid() is the number of blocks launch_hv() uses, i.e. the number
 // of partial sums it writes when dot_partials != null.
 int hv_launch_grid(lsqed_ham* h);
+bool hv_plan_ready(const lsqed_ham* h);
 int launch_hv(lsqed_ham* h, const double* x, double* y, double* dot_partials, hipStream_t s);
 
 // Row-split kernel (hv_split.hip).  The unspecialised kernel handles
