@@ -24,6 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    """Progress line (stderr: stdout carries the one JSON result line)."""
+    print(msg, file=sys.stderr, flush=True)
 METRIC = "Sz=0 H·v GB/s + Lanczos steps/s at N=32; batched realizations/s vs plain"
 
 
@@ -33,16 +38,80 @@ def algorithmic_bytes(N, nup, n_bonds_flip):
     return 8 * (2 * D + nnz), D, nnz
 
 
-def measured_traffic(kernel_name, lx, ly):
-    """HBM bytes per H.v launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE, calibrated against k_lanczos_update's known
-    traffic); null when no measurement of this kernel and lattice exists."""
-    path = os.path.join(ROOT, "profiles", f"hv_traffic_{lx}x{ly}.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hv_traffic.json")
+
+# HBM bytes of one dispatch from the L2 (TCC) -> memory request counters of
+# gfx950: reads are 128-, 64- or 32-byte requests (TCC_EA0_RDREQ_{128B,64B,32B}),
+# writes are counted in 32-byte units going to DRAM (TCC_EA0_WRREQ_WRITE_DRAM_32B).
+# rocprofv3's FETCH_SIZE counts 128-byte requests through TCC_BUBBLE, which
+# reads 0 on gfx950, so it reports half of a streaming read; the request
+# counters above are exact on the probes of `--calibrate` (read stream, triad,
+# and one double per 128-byte line or per 64-byte half line: both cost one
+# 128-byte request per access).
+READ_COUNTERS = {"TCC_EA0_RDREQ_128B_sum": 128, "TCC_EA0_RDREQ_64B_sum": 64, "TCC_EA0_RDREQ_32B_sum": 32}
+WRITE_COUNTERS = {"TCC_EA0_WRREQ_WRITE_DRAM_32B_sum": 32}
+
+
+def measured_traffic(fingerprint):
+    """The counter record (HBM bytes of one launch) of the H.v whose
+    fingerprint (lsqed_ham_hv_fingerprint: kernel, grid, model and the hash
+    of the generated kernel source) is `fingerprint`; None when no record of
+    exactly this kernel exists, so a changed kernel never inherits old bytes."""
     try:
-        with open(path) as fh:
-            return json.load(fh)["kernels"][kernel_name]["hbm_bytes"]
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)["records"].get(fingerprint)
     except (OSError, KeyError, ValueError):
         return None
+
+
+def record_traffic(prefix, kernel="hv_jit_nodot"):
+    """Build the traffic record of one H.v from rocprofv3 --pmc passes
+    `<prefix>_<i>/run_counter_collection.csv` (made with `--hv-only`, whose
+    JSON line in `<prefix>_<i>.log` names the fingerprint) and store it in
+    profiles/hv_traffic.json under that fingerprint."""
+    import csv
+
+    vals, fp, hv = {}, None, None
+    for i in range(1, 33):
+        d = f"{prefix}_{i}"
+        f = os.path.join(d, "run_counter_collection.csv")
+        if not os.path.exists(f):
+            continue
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                if kernel in row["Kernel_Name"]:
+                    vals.setdefault(row["Counter_Name"], []).append(float(row["Counter_Value"]))
+        if os.path.exists(d + ".log"):
+            for ln in open(d + ".log"):
+                if ln.startswith('{"hv"'):
+                    hv = json.loads(ln)["hv"]
+                    fp = fp or hv["fingerprint"]
+                    if hv["fingerprint"] != fp:
+                        raise SystemExit(f"record_traffic: {d} profiled another kernel")
+    mean = {k: sum(v) / len(v) for k, v in vals.items()}
+    missing = [c for c in list(READ_COUNTERS) + list(WRITE_COUNTERS) if c not in mean]
+    if missing or not fp:
+        raise SystemExit(f"record_traffic: missing counters {missing} or fingerprint under {prefix}")
+    rd = sum(mean[c] * w for c, w in READ_COUNTERS.items())
+    wr = sum(mean[c] * w for c, w in WRITE_COUNTERS.items())
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            doc = json.load(fh)
+    except (OSError, ValueError):
+        doc = {}
+    doc.setdefault("method", "HBM bytes per dispatch = 128 TCC_EA0_RDREQ_128B + 64 TCC_EA0_RDREQ_64B + "
+                             "32 TCC_EA0_RDREQ_32B (reads) + 32 TCC_EA0_WRREQ_WRITE_DRAM_32B (writes), "
+                             "mean over the profiled dispatches of the kernel")
+    doc.setdefault("records", {})[fp] = {
+        "workload": hv["workload"], "kernel": kernel, "hv_kernel": hv["hv_kernel"],
+        "read_bytes": int(rd), "write_bytes": int(wr), "hbm_bytes": int(rd + wr),
+        "counters": {k: round(v, 1) for k, v in sorted(mean.items())},
+        "dispatches": max(len(v) for v in vals.values()),
+        "profiled_hv_ms": hv["hv_ms"]}
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(doc, fh, indent=1, sort_keys=True)
+        fh.write("\n")
+    return fp, doc["records"][fp]
 
 
 def cpu_baseline(h_args, target_s=12.0):
@@ -180,7 +249,9 @@ def disorder_batch(reps: int = 3):
         "all_converged": bool(all(i[2] for i in info)),
         "schedule": sched,
         "step_algorithmic_bytes": hv_bytes + upd_bytes,
-        "step_gbs": round((hv_bytes + upd_bytes) / (sched["step_ms"] * 1e-3) / 1e9, 1) if sched["step_ms"] else None,
+        # effective bandwidth: one 8-byte read per gather, cache hits included (not an HBM fraction)
+        "step_effective_gbs": round((hv_bytes + upd_bytes) / (sched["step_ms"] * 1e-3) / 1e9, 1)
+        if sched["step_ms"] else None,
         "plain_realisations_per_s": round(n / t_p, 2),
         "plain_s": round(t_p, 3),
         "plain": "Krylov-Schur nev=1 per realisation (SolverInit/solve/solution/SolverClean, tol 1e-9) with "
@@ -226,6 +297,43 @@ def stream_ceiling(q, p, y, reps: int = 10):
     ms_tt = timed(lambda: torch.add(q[:n], p[:n], alpha=0.5, out=y[:n]))
     return {"read_gbs": round(8 * n / ms_r / 1e6, 1), "triad_gbs": round(24 * n / ms_t / 1e6, 1),
             "torch_triad_gbs": round(24 * n / ms_tt / 1e6, 1), "elements": n}
+
+
+def calibrate(reps: int = 2):
+    """Kernels of known traffic for reading the HBM byte counters (rocprofv3
+    --pmc): the read-only stream (8 B per element read), the STREAM triad
+    (16 B read + 8 B written per element), and the scattered-line probes
+    (one double per 128-byte line, resp. per 64-byte half line, every line
+    touched once in an order with no locality).  Each is launched `reps`
+    times on 2 GiB buffers; prints one JSON object with the algorithmic bytes
+    of one launch of each."""
+    import ctypes as C
+
+    import torch
+
+    from lsquantumed_amd._lib import check, lib
+
+    n = 1 << 28
+    b = torch.rand(n, dtype=torch.float64, device="cuda")
+    c = torch.rand(n, dtype=torch.float64, device="cuda")
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    part = torch.empty(2048, dtype=torch.float64, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    lines, halves = n // 16, n // 8
+    for _ in range(reps):
+        check(lib().lsqed_stream_probe(1, None, P(b), None, n, P(part), st))
+        check(lib().lsqed_stream_probe(0, P(a), P(b), P(c), n, None, st))
+        check(lib().lsqed_stream_probe(2, P(a), P(b), None, lines, None, st))
+        check(lib().lsqed_stream_probe(3, P(a), P(b), None, halves, None, st))
+    torch.cuda.synchronize()
+    return {"k_stream_read": {"read": 8 * n, "write": 0},
+            "k_stream_triad": {"read": 16 * n, "write": 8 * n},
+            "k_stream_gather sh=4 (one double per 128-B line)": {"lines": lines, "useful_read": 8 * lines,
+                                                                 "write": 8 * lines},
+            "k_stream_gather sh=3 (one double per 64-B half line)": {"halves": halves, "useful_read": 8 * halves,
+                                                                     "write": 8 * halves},
+            "launches_each": reps}
 
 
 def cpu_disorder_baseline():
@@ -343,7 +451,7 @@ def hv_4x6_with_cpu():
     cpu_s = (time.perf_counter() - t) / 3
     csr_bytes = 24 * A.nnz + 16 * A.nnz + 16 * D
     return {"workload": "H.v, periodic square 4x6 (N=24), J1=D1=1, J2=D2=0.5, Sz=0, D=2704156",
-            "algorithmic_bytes": B, "gpu_ms": round(gpu_ms, 4), "gpu_gbs": round(B / gpu_ms / 1e6, 1),
+            "algorithmic_bytes": B, "gpu_ms": round(gpu_ms, 4), "gpu_effective_gbs": round(B / gpu_ms / 1e6, 1),
             "gpu_kernel": {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()],
             "cpu_ms": round(cpu_s * 1e3, 2), "cpu_gbs": round(B / cpu_s / 1e9, 2),
             "cpu_csr_gbs": round(csr_bytes / cpu_s / 1e9, 2), "cpu_threads": threads,
@@ -435,7 +543,7 @@ def krylov_schur_4x8(tol: float = 1e-9):
     h = Q.Hamiltonian(env, Q.Basis(env, 0), Q.square2D(env, 4, 8), 1.0, 1.0, 0.5, 0.5)
     h.build_diag(env)
     h.build_off_diag(env)
-    print("4x8 Krylov-Schur nev=4 ncv=20: start", flush=True)
+    log("4x8 Krylov-Schur nev=4 ncv=20: start")
     torch.cuda.synchronize()
     t = time.perf_counter()
     eps = Q.Solver.SolverInit(h, 4, 20, -2, tol=tol)
@@ -445,17 +553,18 @@ def krylov_schur_4x8(tol: float = 1e-9):
     sol = [Q.Solver.solution(eps, n) for n in range(min(nconv, 4))]
     its = eps.iterations()
     Q.Solver.SolverClean(eps)
-    print(f"4x8 Krylov-Schur: {dt:.1f} s, {its} restarts, nconv {nconv}", flush=True)
+    log(f"4x8 Krylov-Schur: {dt:.1f} s, {its} restarts, nconv {nconv}")
     del eps
     torch.cuda.empty_cache()
-    t = time.perf_counter()
-    e0, steps = Q.lanczos_ground(h, maxit=1000, tol=1e-13)
-    dl = time.perf_counter() - t
+    r = Q.solver.lanczos_ground_residual(h, tol=1e-11, maxit=800, seed=5)
+    e0, steps, dl = r["E0"], r["steps"], r["wall_s"]
+    torch.cuda.empty_cache()
     return {"workload": f"Krylov-Schur nev=4 ncv=20 tol {tol:g}, periodic square 4x8 (N=32), J1=D1=1, J2=D2=0.5, "
                         "Sz=0, D=601080390 (basis 21 x 4.81 GB)",
             "solve_s": round(dt, 2), "restarts": its, "nconv": nconv,
             "eigenvalues": [v for v, _ in sol], "relative_errors": [e for _, e in sol],
             "lanczos3_E0": e0, "lanczos3_steps": steps, "lanczos3_s": round(dl, 2),
+            "lanczos3": "three-vector Lanczos to a 1e-11 relative residual, the cross-check of the lowest value",
             "E0_diff_krylov_schur_vs_lanczos3": abs(sol[0][0] - e0) if sol else None}
 
 
@@ -465,9 +574,9 @@ def lanczos_6x6(maxit: int = 600, tol: float = 1e-9):
     Lanczos: q, p and y on the GPU (218 GB), one lsqed_lanczos_step per step
     (H.v with the alpha dot fused, then the fused update and norm), the
     tridiagonal matrix on the host; stops when the lowest Ritz pair has
-    |beta_m s_m| / |theta| < tol.  Prints a progress line every 5 steps."""
-    import numpy as np
-    import scipy.linalg as sl
+    |beta_m s_m| / |theta| < tol.  Progress lines go to stderr every 5 steps.
+    Also times one H.v alone (HIP events) and reports its HBM traffic when the
+    counter record of this kernel exists."""
     import torch
 
     import lsquantumed_amd as Q
@@ -479,48 +588,35 @@ def lanczos_6x6(maxit: int = 600, tol: float = 1e-9):
     h.build_off_diag(env)
     D = h.hamilt_size()
     kernel = {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()]
-    g = torch.Generator(device="cuda").manual_seed(7)
-    q = torch.rand(D, dtype=torch.float64, device="cuda", generator=g)
-    q -= 0.5
-    p = torch.zeros(D, dtype=torch.float64, device="cuda")
+    fp = h.hv_fingerprint()
+    x = torch.empty(D, dtype=torch.float64, device="cuda").uniform_(-0.5, 0.5)
     y = torch.empty(D, dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
-    h.matvec(q, y)  # warm-up (kernel compilation, tables)
+    h.matvec(x, y)  # warm-up (kernel compilation, tables)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    h.matvec(q, y)
+    h.matvec(x, y)
     ev1.record(stream)
     torch.cuda.synchronize()
     hv_ms = ev0.elapsed_time(ev1)
+    del x, y
+    torch.cuda.empty_cache()
     B, _, nnz = algorithmic_bytes(N, N // 2, 144)
-    print(f"6x6: D={D} kernel={kernel} H.v {hv_ms:.1f} ms = {B / hv_ms / 1e6:.1f} GB/s", flush=True)
-    sq = 1.0 / torch.linalg.vector_norm(q).item()
-    sp, beta = 0.0, 0.0
-    al, be = [], []
-    theta, res = None, None
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for m in range(maxit):
-        a, b = Q.lanczos_step(h, q, sq, p, sp, beta, y)
-        al.append(a)
-        be.append(b)
-        q, p = p, q
-        sp, sq, beta = sq, 1.0 / b, b
-        if m % 5 == 4 or m == maxit - 1:
-            w, v = sl.eigh_tridiagonal(np.array(al), np.array(be[:-1]), select="i", select_range=(0, 0))
-            theta, res = float(w[0]), abs(be[-1] * v[-1, 0]) / abs(float(w[0]))
-            print(f"6x6 step {m + 1}: theta {theta:.12f} resid {res:.3e} ({time.perf_counter() - t0:.1f} s)",
-                  flush=True)
-            if res < tol:
-                break
-    total = time.perf_counter() - t0
-    steps = len(al)
+    rec = measured_traffic(fp)
+    log(f"6x6: D={D} kernel={kernel} H.v {hv_ms:.1f} ms, {B / hv_ms / 1e6:.1f} GB/s effective")
+    r = Q.solver.lanczos_ground_residual(
+        h, tol=tol, maxit=maxit, seed=7,
+        progress=lambda m, th, res, sec: log(f"6x6 step {m}: theta {th:.12f} resid {res:.3e} ({sec:.1f} s)"))
+    steps = r["steps"]
     return {"workload": f"ground state, periodic square 6x6 (N=36), J1=D1=1, J2=D2=0.5, Sz=0, D={D}, "
                         f"three-vector Lanczos to a {tol:g} relative residual",
-            "E0": theta, "E0_per_site": theta / N, "steps": steps, "relative_residual": res,
-            "wall_s": round(total, 2), "s_per_step": round(total / steps, 4), "hv_kernel": kernel,
-            "hv_ms": round(hv_ms, 2), "hv_algorithmic_bytes": B, "hv_gbs": round(B / hv_ms / 1e6, 1),
-            "hv_roofline_frac": round(B / hv_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "E0": r["E0"], "E0_per_site": r["E0"] / N, "steps": steps, "relative_residual": r["relative_residual"],
+            "wall_s": round(r["wall_s"], 2), "s_per_step": round(r["wall_s"] / steps, 4),
+            "lanczos_steps_per_s": round(steps / r["wall_s"], 3), "hv_kernel": kernel, "hv_fingerprint": fp,
+            "hv_ms": round(hv_ms, 2), "hv_algorithmic_bytes": B, "hv_effective_gbs": round(B / hv_ms / 1e6, 1),
+            "hv_hbm_bytes": rec["hbm_bytes"] if rec else None,
+            "hv_hbm_gbs": round(rec["hbm_bytes"] / hv_ms / 1e6, 1) if rec else None,
+            "hv_roofline_frac": round(rec["hbm_bytes"] / hv_ms / 1e6 / HBM_PEAK_GBS, 4) if rec else None,
             "device_gb": round(3 * D * 8 / 1e9, 1)}
 
 
@@ -565,12 +661,17 @@ def hv_only(lx: int, ly: int, reps: int, warmup: int, chain: int = 0, hv_kernel:
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / reps
     kernel = {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()]
+    fp = h.hv_fingerprint()
+    rec = measured_traffic(fp)
     name = f"periodic chain N={N}, J1=D1=1, J2=D2=0.2" if chain > 0 else \
         f"periodic square {lx}x{ly} (N={N}), J1=D1=1, J2=D2=0.5"
     return {"workload": f"H.v, {name}, Sz=0, D={D}",
-            "hv_kernel": kernel, "split_low_bits": os.environ.get("LSQED_SPLIT_L", "default"),
+            "hv_kernel": kernel, "fingerprint": fp, "split_low_bits": os.environ.get("LSQED_SPLIT_L", "default"),
             "row_threads": os.environ.get("LSQED_JIT_ROW_THREADS", "default"), "hv_ms": round(ms, 4),
-            "algorithmic_bytes": B, "gbs": round(B / ms / 1e6, 1), "roofline_frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": B, "effective_gbs": round(B / ms / 1e6, 1),
+            "hbm_bytes": rec["hbm_bytes"] if rec else None,
+            "hbm_gbs": round(rec["hbm_bytes"] / ms / 1e6, 1) if rec else None,
+            "roofline_frac": round(rec["hbm_bytes"] / ms / 1e6 / HBM_PEAK_GBS, 4) if rec else None,
             "x_dot_hx": chk}
 
 
@@ -595,7 +696,18 @@ def main():
     ap.add_argument("--hv-only", action="store_true",
                     help="only H.v on the --lx x --ly lattice with two vectors (6x6: 145 GB); one JSON object")
     ap.add_argument("--chain", type=int, default=0, help="with --hv-only: a periodic chain of this many spins")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only the kernels of known traffic used to read the HBM byte counters (for rocprofv3 --pmc)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 4x8 Krylov-Schur (nev=4, ncv=20) and the 6x6 Lanczos legs of the default run")
+    ap.add_argument("--record-traffic", metavar="PREFIX",
+                    help="store the H.v traffic record built from rocprofv3 --pmc passes PREFIX_<i>/ (made with "
+                         "--hv-only) in profiles/hv_traffic.json, keyed by the kernel fingerprint; no GPU use")
     args = ap.parse_args()
+    if args.record_traffic:
+        fp, rec = record_traffic(args.record_traffic)
+        print(json.dumps({"fingerprint": fp, **rec}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Sz-sector path here is built and measured on a single "
               "MI355X (multi-GPU sharding is design-only).", file=sys.stderr)
@@ -606,6 +718,9 @@ def main():
     import lsquantumed_amd as Q
 
     torch.cuda.set_device(0)
+    if args.calibrate:
+        print(json.dumps({"calibrate": calibrate()}))
+        return
     if args.batch_only:
         print(json.dumps({"batched_disorder": disorder_batch()}))
         return
@@ -631,6 +746,7 @@ def main():
     if args.hv_kernel:
         h.set_hv_kernel(args.hv_kernel)
     kernel_name = {1: "generic", 2: "row split", 3: "row split (hipRTC)"}[h.hv_kernel()]
+    fingerprint = h.hv_fingerprint()
     nflip = len(lat.nn_bonds[0]) + len(lat.nnn_bonds[0])
     B, D, nnz = algorithmic_bytes(N, N // 2, nflip)
     assert D == h.hamilt_size()
@@ -655,7 +771,8 @@ def main():
     torch.cuda.synchronize()
     hv_ms = ev0.elapsed_time(ev1) / args.steps
     hv_gbs = B / (hv_ms * 1e-3) / 1e9
-    traffic = measured_traffic(kernel_name, lx, ly)
+    rec = measured_traffic(fingerprint)
+    traffic = rec["hbm_bytes"] if rec else None
 
     # --- Lanczos steps ------------------------------------------------------
     sp, beta = 0.0, 0.0
@@ -689,7 +806,14 @@ def main():
     c1 = None if args.no_batch else chain16()
     ag = None if args.no_batch else autograd_chain20()
     hv46 = None if (args.no_batch or args.no_cpu) else hv_4x6_with_cpu()
+    del h
+    torch.cuda.empty_cache()
+    ks48 = None if args.no_large else krylov_schur_4x8()
+    torch.cuda.empty_cache()
+    l66 = None if args.no_large else lanczos_6x6()
+    torch.cuda.empty_cache()
     cpu = None if args.no_cpu else cpu_baseline((N, N // 2, J1, D1, J2, D2, lx, ly, B))
+    hbm_gbs = traffic / (hv_ms * 1e-3) / 1e9 if traffic else None
     out = {
         "metric": METRIC,
         "value": round(hv_gbs, 2),
@@ -710,11 +834,19 @@ def main():
         "hv_ms": round(hv_ms, 3),
         "lanczos_steps_per_s": round(steps_per_s, 3),
         "lanczos_wall_s": round(wall, 3),
-        "roofline": {"bound": "hbm", "achieved": round(hv_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(hv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     # measured HBM bytes per launch over the same launch time (the algorithmic bytes
-                     # count every gather, also those served from cache)
-                     "traffic_gbs": round(traffic / (hv_ms * 1e-3) / 1e9, 1) if traffic else None,
+        # achieved / frac: the HBM bytes one launch of this exact kernel moves (hardware counters of
+        # the committed record whose fingerprint matches, profiles/hv_traffic.json) over the launch
+        # time measured here; null when no record of this kernel exists.  The algorithmic figure
+        # counts one 8-byte read per gather, also those served by a cache, so it can exceed the
+        # peak: it is `value`, the effective bandwidth, not a roofline fraction.
+        "roofline": {"bound": "hbm", "achieved": round(hbm_gbs, 1) if hbm_gbs else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None,
+                     "traffic": traffic, "kernel": "hv_jit_nodot" if kernel_name == "row split (hipRTC)" else kernel_name,
+                     "fingerprint": fingerprint,
+                     "traffic_read_bytes": rec["read_bytes"] if rec else None,
+                     "traffic_write_bytes": rec["write_bytes"] if rec else None,
+                     "algorithmic_bytes": B, "effective_gbs": round(hv_gbs, 1),
+                     "frac_of_measured_read_stream": round(hbm_gbs / stream["read_gbs"], 4) if hbm_gbs else None,
                      "measured_read_stream_gbs": stream["read_gbs"],
                      "measured_triad_gbs": max(stream["triad_gbs"], stream["torch_triad_gbs"])},
         "stream": stream,
@@ -722,6 +854,8 @@ def main():
         "batched_disorder": batch,
         "krylov_schur_4x6": ks,
         "hv_4x6": hv46,
+        "krylov_schur_4x8": ks48,
+        "lanczos_6x6": l66,
         "chain16": c1,
         "autograd_chain20": ag,
         "alpha_last": alphas[-1],

@@ -131,6 +131,14 @@ int lsqed_hv_c128(lsqed_ham* h, const double* x, double* y, lsqed_stream_t strea
 int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant);
 /* The kernel lsqed_hv will use (1, 2 or 3), resolving automatic mode. */
 int lsqed_ham_get_hv_kernel(lsqed_ham* h);
+/* A 64-bit fingerprint of the H.v that lsqed_hv launches: kernel, grid and
+ * model, and for the specialised kernel the hash of its generated source
+ * (bonds, couplings, build mode, tuning) and block size.  No counterpart in
+ * the reference; measurement records (hardware counters of one launch) are
+ * keyed by it, so that a changed kernel does not inherit an old record.  For
+ * the built-in kernels (1, 2) it covers the launch and the model, not the
+ * compiled code. */
+int lsqed_ham_hv_fingerprint(lsqed_ham* h, uint64_t* fp);
 
 /* Diagonal of H exactly as the reference stores it (MatGetDiagonal,
  * tests/unit_tests/build_diag/chain_1d/test_build_diag60.cpp:29): entries the
@@ -311,7 +319,13 @@ int lsqed_batch_destroy(lsqed_batch* b);
 /* Streaming-bandwidth probes, no counterpart in the reference: the measured
  * ceiling reported beside the nominal HBM peak.  kind 0: STREAM triad
  * a = b + 0.5 c (24 B per element); kind 1: read-only stream of b (8 B per
- * element; n even; part: 2048 doubles of block sums).  Device buffers. */
+ * element; n even; part: 2048 doubles of block sums).  kinds 2 and 3
+ * calibrate the HBM byte counters on scattered reads: a[i] = the first double
+ * of piece (i * 1000003 mod n) of b, pieces of 16 doubles (kind 2: one per
+ * 128-byte line, b holds 16 n doubles) or 8 doubles (kind 3: one per 64-byte
+ * half line, b holds 8 n doubles); n < 2^34 and not a multiple of 1000003
+ * (prime), so every piece is read once.
+ * Device buffers. */
 int lsqed_stream_probe(int kind, double* a, const double* b, const double* c, int64_t n, double* part,
                        lsqed_stream_t stream);
 

@@ -58,6 +58,7 @@ SIGNATURES = {
     "lsqed_ham_set_fields": (_i32, [_vp, _vp]),
     "lsqed_hv": (_i32, [_vp, _vp, _vp, _vp]),
     "lsqed_hv_c128": (_i32, [_vp, _vp, _vp, _vp]),
+    "lsqed_ham_hv_fingerprint": (_i32, [_vp, _P(_u64)]),
     "lsqed_diag_c128": (_i32, [_vp, _vp, _vp]),
     "lsqed_ham_set_hv_kernel": (_i32, [_vp, _i32]),
     "lsqed_ham_get_hv_kernel": (_i32, [_vp]),

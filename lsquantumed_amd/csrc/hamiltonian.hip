@@ -562,6 +562,28 @@ int lsqed_hv_c128(lsqed_ham* h, const double* x, double* y, lsqed_stream_t strea
 
 int lsqed_ham_get_hv_kernel(lsqed_ham* h) { return h ? hv_kind(h) : 0; }
 
+int lsqed_ham_hv_fingerprint(lsqed_ham* h, uint64_t* fp) {
+  if (!h || !fp) return fail(LSQED_ERR_ARG, "null argument");
+  const int k = hv_kind(h);
+  const int grid = hv_launch_grid(h);
+  if (grid <= 0) return fail(LSQED_ERR_HIP, "H.v plan setup failed");
+  uint64_t f = fnv1a_val(k, fnv1a_val(grid, fnv1a_val(h->N, fnv1a_val(h->nup, fnv1a_val(h->build, 1469598103934665603ull)))));
+  if (k == 3) {
+    // the generated source holds the bonds, couplings, build mode and tuning
+    f = fnv1a_val(jit_source_hash(h), fnv1a_val(jit_threads(h), f));
+  } else {
+    for (int kind = 0; kind < 2; ++kind) {
+      f = fnv1a(h->bi[kind].data(), h->bi[kind].size() * sizeof(int), f);
+      f = fnv1a(h->bj[kind].data(), h->bj[kind].size() * sizeof(int), f);
+      f = fnv1a_val(h->J[kind], fnv1a_val(h->Dz[kind], f));
+    }
+    const char* v = lsqed_version();
+    f = fnv1a(v, std::strlen(v), f);
+  }
+  *fp = f;
+  return LSQED_OK;
+}
+
 int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant) {
   if (!h) return fail(LSQED_ERR_ARG, "null handle");
   if (variant < 0 || variant > 3)

@@ -541,6 +541,7 @@ struct JitModule {
   bool ok = false;
   bool rowstage = false;  // one block per row (rows_ord), else 256 threads per item
   int threads = 256;      // block size
+  uint64_t src_hash = 0;  // FNV-1a of the generated source
   std::string log;
 };
 
@@ -664,6 +665,7 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx) {
 
 static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch) {
   auto m = std::make_shared<JitModule>();
+  m->src_hash = fnv1a(src.data(), src.size());
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "lsqed_hv_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
     m->log = "hiprtcCreateProgram failed";
@@ -760,6 +762,9 @@ int jit_grid(lsqed_ham* h) {
   if (jit_prepare(h) != LSQED_OK) return 0;
   return jit_grid_of(h, (const JitModule*)h->jit);
 }
+
+uint64_t jit_source_hash(const lsqed_ham* h) { return h->jit ? ((const JitModule*)h->jit)->src_hash : 0; }
+int jit_threads(const lsqed_ham* h) { return h->jit ? ((const JitModule*)h->jit)->threads : 0; }
 
 // The complex128 module of the handle (same bond structure and couplings).
 static int jit_prepare_c128(lsqed_ham* h) {

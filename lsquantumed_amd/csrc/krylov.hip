@@ -403,6 +403,17 @@ __global__ __launch_bounds__(kThreads) void k_stream_triad(double* __restrict__ 
     a[i] = b[i] + s * c[i];
 }
 
+// Scattered-line probes (calibrate the HBM byte counters on a gather): entry
+// i reads one double at the start of 2^sh-double piece ((i * P) mod m) of b,
+// m = n pieces, so every piece of b is touched exactly once, in an order with
+// no locality (P = 1000003 is prime, so coprime with any m it does not divide).  sh = 4: one double
+// per 128-byte line; sh = 3: one per 64-byte half line.
+__global__ __launch_bounds__(kThreads) void k_stream_gather(double* __restrict__ a, const double* __restrict__ b,
+                                                            int64_t n, int sh) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = b[((i * 1000003) % n) << sh];
+}
+
 static int red_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (n + 255) / 256)); }
 static int stream_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256)); }
 
@@ -1053,8 +1064,12 @@ int lsqed_stream_probe(int kind, double* a, const double* b, const double* c, in
   } else if (kind == 1) {
     if (!part || (n & 1)) return fail(LSQED_ERR_ARG, "read stream needs an even n and 2048 partials");
     hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(kThreads), 0, s, b, n / 2, part);
+  } else if (kind == 2 || kind == 3) {
+    if (!a || n % 1000003 == 0 || n >= ((int64_t)1 << 34))
+      return fail(LSQED_ERR_ARG, "gather probe needs a, n < 2^34 and n not a multiple of 1000003");
+    hipLaunchKernelGGL(k_stream_gather, dim3(1024), dim3(kThreads), 0, s, a, b, n, kind == 2 ? 4 : 3);
   } else {
-    return fail(LSQED_ERR_ARG, "kind must be 0 (triad) or 1 (read)");
+    return fail(LSQED_ERR_ARG, "kind must be 0 (triad), 1 (read), 2 or 3 (gathers)");
   }
   LSQED_HIP(hipGetLastError());
   return LSQED_OK;

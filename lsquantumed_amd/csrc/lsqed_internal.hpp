@@ -49,6 +49,15 @@ void build_sector_tables(int N, int nup, SectorTables& t);
 uint64_t binom_u64(int n, int k);
 int64_t sector_dim(int N, int nup);
 
+// FNV-1a over bytes (fingerprints of generated kernel sources and launch shapes).
+inline uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* p = (const unsigned char*)data;
+  for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+  return h;
+}
+template <typename T>
+inline uint64_t fnv1a_val(const T& v, uint64_t h) { return fnv1a(&v, sizeof(T), h); }
+
 // Packed bond: lo | hi << 8 (lo < hi, positions of the two sites).
 inline uint32_t pack_bond(int a, int b) {
   int lo = a < b ? a : b, hi = a < b ? b : a;
@@ -158,6 +167,8 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
 int launch_hv_jit_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s);
 int launch_hv_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s);
 int jit_grid(lsqed_ham* h);
+uint64_t jit_source_hash(const lsqed_ham* h);  // of the prepared module's generated source (0: none)
+int jit_threads(const lsqed_ham* h);
 
 // Batch variant of the specialised kernel (hv_jit.cpp): problems sharing a
 // bond structure and build mode form a group; one launch covers the group.

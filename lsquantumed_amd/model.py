@@ -265,6 +265,16 @@ class Hamiltonian:
         """The kernel matvec uses (1 generic, 2 row split, 3 specialised row split)."""
         return int(lib().lsqed_ham_get_hv_kernel(self.hamilt))
 
+    def hv_fingerprint(self) -> str:
+        """16 hex digits identifying the H.v launch (kernel, grid, model and,
+        for the specialised kernel, its generated source); measurement
+        records are keyed by it."""
+        import ctypes as C
+
+        v = C.c_uint64()
+        check(lib().lsqed_ham_hv_fingerprint(self.hamilt, C.byref(v)))
+        return f"{v.value:016x}"
+
     def matvec(self, x, y=None, stream=None):
         """y = H x on the GPU (x, y: float64 CUDA tensors of length D)."""
         import torch
