@@ -6,8 +6,8 @@ of include/lsqed.h; PyTorch only provides device memory and streams.
 """
 from ._lib import BUILD_DIAG, BUILD_DISORDER, BUILD_OFFDIAG, LsqedError, lib  # noqa: F401
 from .model import AF, Basis, Environment, Hamiltonian, Striped, chain1D, honeycomb2D, square2D  # noqa: F401
-from .phys import (Phys, SiSj, SmSp, SpSm, SzSz, correlations, disorder_averaged_spectrum, dsf_spectrum,  # noqa: F401
-                   dsf_weights, magnetisation, multiplet_weights, q_grid, read_dsf, site_coords)
+from .phys import (Phys, SiSj, SmSp, SpSm, SzSz, correlations, dsf_weights, magnetisation,  # noqa: F401
+                   multiplet_weights, q_grid, read_dsf, site_coords)
 from .solver import EPS, PETSC_DECIDE, PETSC_DEFAULT, Solver, lanczos_ground, lanczos_step  # noqa: F401
 from .tools import RandomDisorder, fields_from_seed  # noqa: F401
 from .batch import BatchSolver, disorder_chains  # noqa: F401

@@ -113,3 +113,106 @@ def mat_view(h, nprocs: int = 1, stream=None) -> str:
     for r, ents in enumerate(mat_rows(h, stream=stream)):
         out.append(f"row {r}:" + "".join(f" ({c}, {petsc_g(v)}) " for c, v in ents))
     return "\n".join(out) + "\n"
+
+
+# ------------------------------------------------------------------ import --
+# The reverse direction: a VecView / MatView ASCII dump (the files the
+# reference's tests write and compare.sh diffs) read back into the library's
+# vectors.  Values are PETSc's %g text (6 significant digits), so a loaded
+# dump reproduces its own text exactly and the vector it was printed from to
+# %g precision.
+
+def _text(src) -> str:
+    if isinstance(src, str) and ("\n" in src or src.startswith(("Vec Object", "Mat Object"))):
+        return src
+    with open(src) as fh:
+        return fh.read()
+
+
+def parse_scalar(tok: str) -> complex:
+    """One PETSc ASCII scalar: %g with an optional forced '.', or a complex
+    value written as 're + im i' / 're - im i' (complex-scalar builds)."""
+    t = tok.strip()
+    if t.endswith("i") and (" + " in t or " - " in t):
+        sep = " + " if " + " in t else " - "
+        re_s, im_s = t[:-1].rsplit(sep, 1)
+        im = float(im_s.strip())
+        return complex(float(re_s), im if sep == " + " else -im)
+    return complex(float(t), 0.0)
+
+
+def read_vec(src) -> tuple[np.ndarray, int]:
+    """(values, number of ranks) of a VecView dump: 'Vec Object: <n> MPI
+    processes', the type line, then one value per line, in 'Process [k]'
+    blocks for an MPI vector.  Values are float64, or complex128 when any has
+    an imaginary part."""
+    lines = _text(src).splitlines()
+    if not lines or not lines[0].startswith("Vec Object:"):
+        raise ValueError("not a PETSc VecView dump (no 'Vec Object:' header)")
+    nprocs = int(lines[0].split(":", 1)[1].split()[0])
+    vals = [parse_scalar(ln) for ln in lines[1:]
+            if ln.strip() and not ln.startswith(("  type", "Process ["))]
+    arr = np.array(vals, dtype=np.complex128)
+    if not np.any(arr.imag):
+        arr = arr.real.copy()
+    return arr, nprocs
+
+
+def read_mat(src) -> tuple[np.ndarray, np.ndarray, np.ndarray, int]:
+    """(row pointer, columns, values, number of ranks) of a MatView dump of
+    an AIJ matrix: 'row i: (c, v)  (c, v) ...' per row, entries as stored."""
+    lines = _text(src).splitlines()
+    if not lines or not lines[0].startswith("Mat Object:"):
+        raise ValueError("not a PETSc MatView dump (no 'Mat Object:' header)")
+    nprocs = int(lines[0].split(":", 1)[1].split()[0])
+    rp, cols, vals = [0], [], []
+    for ln in lines[1:]:
+        if not ln.startswith("row "):
+            continue
+        head, body = ln.split(":", 1)
+        if int(head.split()[1]) != len(rp) - 1:
+            raise ValueError(f"rows out of order at '{head}'")
+        for part in body.split(")"):
+            part = part.strip()
+            if not part:
+                continue
+            c, v = part.lstrip("(").split(",", 1)
+            cols.append(int(c))
+            vals.append(parse_scalar(v))
+        rp.append(len(cols))
+    v = np.array(vals, dtype=np.complex128)
+    if not np.any(v.imag):
+        v = v.real.copy()
+    return np.array(rp, np.int64), np.array(cols, np.int64), v, nprocs
+
+
+def vec_load(src, device="cuda"):
+    """A VecView dump (text or path) as a vector on the device: float64 (or
+    complex128) CUDA tensor in basis-rank order, ready for Hamiltonian.matvec,
+    the solver's start or a correlator.  Returns (tensor, number of ranks of
+    the dump)."""
+    import torch
+
+    v, nprocs = read_vec(src)
+    return torch.from_numpy(v).to(device), nprocs
+
+
+def mat_load(src, device="cuda"):
+    """A MatView dump (text or path) as a CSR matrix on the device
+    (torch.sparse_csr_tensor, float64 or complex128, int64 indices) and the
+    number of ranks of the dump."""
+    import torch
+
+    rp, cols, vals, nprocs = read_mat(src)
+    D = len(rp) - 1
+    A = torch.sparse_csr_tensor(torch.from_numpy(rp), torch.from_numpy(cols), torch.from_numpy(vals), size=(D, D))
+    return A.to(device), nprocs
+
+
+def mat_view_csr(rp, cols, vals, nprocs: int = 1) -> str:
+    """MatView text of CSR arrays (the inverse of read_mat for real values)."""
+    out = [f"Mat Object: {_procs(nprocs)}", "  type: mpiaij"]
+    for r in range(len(rp) - 1):
+        out.append(f"row {r}:" + "".join(f" ({int(cols[q])}, {petsc_g(float(vals[q]))}) "
+                                         for q in range(rp[r], rp[r + 1])))
+    return "\n".join(out) + "\n"

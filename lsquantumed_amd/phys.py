@@ -222,30 +222,3 @@ def multiplet_weights(delta_e, W, tol: float = 1e-8):
             cols.append(W[:, start:n].sum(axis=1))
             start = n
     return np.array(levels), np.stack(cols, 1)
-
-
-def dsf_spectrum(delta_e, W, omega, eps: float) -> np.ndarray:
-    """S(q, w) = sum_n W[q, n] L(w - (E_n - E_0)) of one realisation with the
-    notebook's broadening L(x) = eps^2 / (pi (eps^2 + x^2)) (cells 11, 15)."""
-    e = np.asarray(delta_e, dtype=float)
-    w = np.asarray(omega, dtype=float)
-    L = (eps * eps / np.pi) / (eps * eps + (w[:, None] - e[None, :]) ** 2)  # [nw, nev]
-    return np.asarray(W, dtype=float) @ L.T  # [nq, nw]
-
-
-def disorder_averaged_spectrum(paths, lx: int, ly: int = 1, omega=None, eps: float | None = None):
-    """The notebook's S(q, w): per realisation file the q-resolved weights
-    and broadened spectrum, summed, times 2 pi, divided by the number of
-    realisations (cells 9-17).  Defaults: w in [0, 5] on 5001 points,
-    eps = 1 / N.  Returns (omega, S[nq, nw])."""
-    N = lx * ly
-    w = np.linspace(0.0, 5.0, 5001) if omega is None else np.asarray(omega, dtype=float)
-    eps = 1.0 / N if eps is None else eps
-    acc = None
-    for p in paths:
-        de, amp, _ = read_dsf(p)
-        S = dsf_spectrum(de, dsf_weights(amp, lx, ly), w, eps)
-        acc = S if acc is None else acc + S
-    if acc is None:
-        raise ValueError("no realisation files")
-    return w, acc * 2.0 * np.pi / len(paths)

@@ -65,3 +65,33 @@ def test_build_hamilt_dumps_exact(torch, refdata, tag):
             if petsc_io.mat_view(h, P) == ref:
                 hits.append((lat, J2))
     assert hits, tag
+
+
+def test_load_dumps_into_device_vectors(torch, literals, refdata):
+    """The import half: each of the 31 compare dumps read back onto the
+    device (vec_load / mat_load) equals the library's own operator to the
+    dumps' %g precision -- the diagonal for VecView dumps, y = A x against
+    the library's H.v for MatView dumps -- and prints back to the same text
+    from device memory."""
+    from lsquantumed_amd import petsc_io
+    for c in literals["compare"]:
+        say("load", c["ref"])
+        hi = np.arange(c["N"]) / c["N"] if c["hi"] else None
+        h = make(c["lattice"], c["N"], c["mag"], c["J1"], c["D1"], c["J2"], c["D2"], c["lx"], c["ly"],
+                 builds=tuple(c["builds"]), hi=hi)
+        ref = refdata(c["ref"])
+        if c["view"] == "vec":
+            v, P = petsc_io.vec_load(ref)
+            assert v.is_cuda and v.dtype == torch.float64 and v.numel() == h.hamilt_size()
+            d = h.diagonal()
+            assert torch.max(torch.abs(v - d)).item() <= 5e-6 * max(1.0, torch.max(torch.abs(d)).item())
+            assert petsc_io.vec_view(v.cpu().numpy(), P) == ref
+        else:
+            A, P = petsc_io.mat_load(ref)
+            assert A.is_cuda and A.layout == torch.sparse_csr and A.shape[0] == h.hamilt_size()
+            x = torch.rand(h.hamilt_size(), dtype=torch.float64, device="cuda")
+            y = torch.mv(A, x)
+            yref = h.matvec(x)
+            assert torch.max(torch.abs(y - yref)).item() <= 5e-6 * max(1.0, torch.max(torch.abs(yref)).item())
+            rp, cols, vals = (t.cpu().numpy() for t in (A.crow_indices(), A.col_indices(), A.values()))
+            assert petsc_io.mat_view_csr(rp, cols, vals, P) == ref

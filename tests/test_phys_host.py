@@ -1,10 +1,10 @@
 """CPU checks of the DSF post-processing of the Python host mirror
 (lsquantumed_amd.phys: read_dsf, site_coords, q_grid, dsf_weights,
-multiplet_weights, dsf_spectrum, disorder_averaged_spectrum), which restates
-postprocessing/dynamicStep0_preprocess.ipynb cells 3-4, 9, 11-17 on the
+multiplet_weights), which restates the q transform of
+postprocessing/dynamicStep0_preprocess.ipynb cells 3-4, 9, 12-13 on the
 files Phys::DynStructFactor writes.  Pinned by the reference's DSF fixture
 (tests/unit_tests/reference_data/dsf_d0_1.ref), by closed forms of the
-lattice Fourier sum and of the broadening integral, and by the oracle's
+lattice Fourier sum, and by the oracle's
 dsf_weight / multiplets on the same amplitudes."""
 import os
 
@@ -65,25 +65,3 @@ def test_weights_and_multiplets_vs_oracle(O):
     for j, g in enumerate(groups):
         assert abs(levels[j] - de[g[0]]) == 0.0
         np.testing.assert_allclose(Wl[:, j], W[:, g].sum(axis=1), rtol=0, atol=1e-15)
-
-
-def test_spectrum_integral_and_average():
-    """The broadening eps^2 / (pi (eps^2 + x^2)) integrates to
-    (eps / pi)(atan(b / eps) - atan(a / eps)) over [a, b] (eps over the real
-    line), so the integral over w of S(q, w) is that sum over the levels
-    weighted by W[q, n]; two copies of one file average to 2 pi times that
-    file's spectrum."""
-    from lsquantumed_amd import phys
-    path = os.path.join(REF, "dsf_d0_1.ref")
-    de, amp, _ = phys.read_dsf(path)
-    W = phys.dsf_weights(amp, 6, 1)
-    eps = 0.05
-    w = np.linspace(-400.0, 400.0, 1_600_001)
-    S = phys.dsf_spectrum(de, W, w, eps)
-    integ = np.trapezoid(S, w, axis=1)
-    exact = (eps / np.pi) * (np.arctan((w[-1] - de) / eps) - np.arctan((w[0] - de) / eps))
-    np.testing.assert_allclose(integ, W @ exact, rtol=1e-8, atol=1e-14)
-    np.testing.assert_allclose(integ, eps * W.sum(axis=1), rtol=1e-4)
-    w2, avg = phys.disorder_averaged_spectrum([path, path], 6, 1)
-    one = phys.dsf_spectrum(de, W, w2, 1.0 / 6)
-    np.testing.assert_allclose(avg, 2.0 * np.pi * one, rtol=1e-14, atol=1e-14)
