@@ -36,6 +36,12 @@ static const char* kJitBody = R"JIT(
 #ifndef ROWSTAGE
 #define ROWSTAGE 0
 #endif
+#ifndef NT_STORE
+#define NT_STORE 0
+#endif
+#ifndef NT_STAGE
+#define NT_STAGE 0
+#endif
 // ROWSTAGE: blocks of ROW_THREADS threads take whole rows, staging each
 // row's x in LDS, so that the low bonds (targets inside the row) read LDS
 // instead of issuing global gathers
@@ -275,7 +281,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
     for (int u = 0; u < kStage; ++u) {
       const int e = tid + u * kThreads < R ? tid + u * kThreads : R - 1;
       stage[u] = xzero();
+#if NT_STAGE && !CPLX
+      if (u * kThreads < R) stage[u] = __builtin_nontemporal_load(&xv[B + (BT)e]);
+#else
       if (u * kThreads < R) stage[u] = xv[B + (BT)e];
+#endif
     }
     __syncthreads();  // every wave is done with the previous row's x
 #pragma unroll
@@ -454,7 +464,11 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       const XV xr = xv[r];
 #endif
       const XV yr = xdiag(d, xr, acc);
+#if NT_STORE && !CPLX
+      __builtin_nontemporal_store(yr, &yv[r]);
+#else
       yv[r] = yr;
+#endif
       if (DOT) dotacc += xdot(xr, yr);
     }
 #if BATCH
@@ -637,6 +651,10 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx) {
   o << "#define GATHER_BATCH " << gather_batch << "\n";
   if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
     if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
+  if (const char* w = getenv("LSQED_JIT_NT_STORE"))
+    if (atoi(w) == 1) o << "#define NT_STORE 1\n";
+  if (const char* w = getenv("LSQED_JIT_NT_STAGE"))
+    if (atoi(w) == 1) o << "#define NT_STAGE 1\n";
   if (jit_rowstage())
     o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L)
       << "\nconstexpr int RMAX = " << binom_u64(L, L / 2) << ";\n";

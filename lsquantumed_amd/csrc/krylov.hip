@@ -351,15 +351,66 @@ __global__ __launch_bounds__(kThreads) void k_restart(double* __restrict__ V, in
   }
 }
 
-// Three-term Lanczos update: p = a*y + b*q + c*p ; part = partial ||p||^2
-__global__ __launch_bounds__(kThreads) void k_lanczos_update(const double* __restrict__ y,
-                                                             const double* __restrict__ q, double* __restrict__ p,
-                                                             int64_t n, double a, double b, double c,
-                                                             double* __restrict__ part) {
+// Three-term Lanczos update: p <- a y + b q + c p and partials of ||p||^2: per block-iteration a tile of
+// 4 double2 per lane (all 12 loads in flight), tiles dealt grid-stride.
+// On the 4x8 sector (D = 601,080,390; 32 B per row) at 1024 blocks: 3.65 ms
+// (5.27 TB/s) against 3.96 ms for one double per lane over contiguous block
+// ranges; 1 or 2 double2 per lane, 2048-65536 blocks and contiguous ranges
+// with double2 measured 3.59-4.28 ms.
+constexpr int kUpdU = 4;
+// The same update for vectors that are not 16-byte aligned (an offset into a
+// workspace of odd length): one double per lane over contiguous block ranges.
+__global__ __launch_bounds__(kThreads) void k_lanczos_update_unaligned(const double* __restrict__ y,
+                                                                       const double* __restrict__ q,
+                                                                       double* __restrict__ p, int64_t n, double a,
+                                                                       double b, double c, double* __restrict__ part) {
   int64_t lo, hi;
   block_range(n, lo, hi);
   double acc = 0.0;
   for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    const double v = a * y[r] + b * q[r] + c * p[r];
+    p[r] = v;
+    acc += v * v;
+  }
+  double t = bsum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lanczos_update(const double* __restrict__ y,
+                                                             const double* __restrict__ q, double* __restrict__ p,
+                                                             int64_t n, double a, double b, double c,
+                                                             double* __restrict__ part) {
+  const double2* __restrict__ Y = reinterpret_cast<const double2*>(y);
+  const double2* __restrict__ Q = reinterpret_cast<const double2*>(q);
+  double2* __restrict__ P = reinterpret_cast<double2*>(p);
+  const int64_t n2 = n / 2, tile = (int64_t)kThreads * kUpdU;
+  double acc = 0.0;
+  for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < n2; t0 += (int64_t)gridDim.x * tile) {
+    double2 vy[kUpdU], vq[kUpdU], vp[kUpdU];
+#pragma unroll
+    for (int u = 0; u < kUpdU; ++u) {
+      const int64_t i = t0 + u * kThreads + threadIdx.x;
+      if (i < n2) {
+        vy[u] = Y[i];
+        vq[u] = Q[i];
+        vp[u] = P[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUpdU; ++u) {
+      const int64_t i = t0 + u * kThreads + threadIdx.x;
+      if (i < n2) {
+        double2 v;
+        v.x = a * vy[u].x + b * vq[u].x + c * vp[u].x;
+        v.y = a * vy[u].y + b * vq[u].y + c * vp[u].y;
+        P[i] = v;
+        acc += v.x * v.x;
+        acc += v.y * v.y;
+      }
+    }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {  // the odd last entry
+    const int64_t r = n - 1;
     const double v = a * y[r] + b * q[r] + c * p[r];
     p[r] = v;
     acc += v * v;
@@ -1093,7 +1144,13 @@ static int lanczos_step_core(lsqed_ham* h, const double* q, double sq, double* p
   double qy = 0.0;
   if ((rc = finish_reduce(h, dpart, G, 1, outv, &qy, s))) return rc;
   const double a = sq * sq * qy;
-  hipLaunchKernelGGL(k_lanczos_update, dim3(R), dim3(kThreads), 0, s, y, q, p, D, sq, -a * sq, -beta * sp, upart);
+  const bool aligned = ((reinterpret_cast<uintptr_t>(y) | reinterpret_cast<uintptr_t>(q) |
+                         reinterpret_cast<uintptr_t>(p)) & 15) == 0;
+  if (aligned)
+    hipLaunchKernelGGL(k_lanczos_update, dim3(R), dim3(kThreads), 0, s, y, q, p, D, sq, -a * sq, -beta * sp, upart);
+  else
+    hipLaunchKernelGGL(k_lanczos_update_unaligned, dim3(R), dim3(kThreads), 0, s, y, q, p, D, sq, -a * sq,
+                       -beta * sp, upart);
   LSQED_HIP(hipGetLastError());
   double ss = 0.0;
   if ((rc = finish_reduce(h, upart, R, 1, outv + 1, &ss, s))) return rc;

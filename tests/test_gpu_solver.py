@@ -296,7 +296,20 @@ def test_stream_probe(torch):
     tot = b.sum().item()
     assert abs(part.sum().item() - tot) <= 1e-12 * tot
     assert lib().lsqed_stream_probe(1, None, ptr(b), None, n - 1, ptr(part), None) == -1
-    assert lib().lsqed_stream_probe(2, ptr(a), ptr(b), ptr(c), n, None, None) == -1
+    assert lib().lsqed_stream_probe(9, ptr(a), ptr(b), ptr(c), n, None, None) == -1
+    # scattered-line probes: a[i] = b[(i * 1000003 mod m) * 16] (kind 2, b of
+    # 16 m doubles) and b[(i * 1000003 mod m) * 8] (kind 3, b of 8 m doubles)
+    m = 4099
+    idx = (np.arange(m, dtype=np.int64) * 1000003) % m
+    assert len(set(idx.tolist())) == m
+    for kind, piece in ((2, 16), (3, 8)):
+        bb = torch.rand(piece * m, dtype=torch.float64, device="cuda", generator=g)
+        aa = torch.zeros(m, dtype=torch.float64, device="cuda")
+        assert lib().lsqed_stream_probe(kind, ptr(aa), ptr(bb), None, m, None, None) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(aa.cpu(), bb.cpu()[torch.from_numpy(idx * piece)])
+    # a piece count that 1000003 divides would revisit pieces: refused
+    assert lib().lsqed_stream_probe(2, ptr(a), ptr(b), None, 1000003, None, None) == -1
 
 
 def test_solver_errors(torch):
