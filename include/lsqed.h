@@ -132,8 +132,9 @@ int lsqed_ham_set_hv_kernel(lsqed_ham* h, int variant);
 /* The kernel lsqed_hv will use (1, 2 or 3), resolving automatic mode. */
 int lsqed_ham_get_hv_kernel(lsqed_ham* h);
 /* A 64-bit fingerprint of the H.v that lsqed_hv launches: kernel, grid and
- * model, and for the specialised kernel the hash of its generated source
- * (bonds, couplings, build mode, tuning) and block size.  No counterpart in
+ * model, the order in which rows are dealt to blocks (row-split kernels), and
+ * for the specialised kernel the hash of its generated source (bonds,
+ * couplings, build mode, tuning) and block size.  No counterpart in
  * the reference; measurement records (hardware counters of one launch) are
  * keyed by it, so that a changed kernel does not inherit an old record.  For
  * the built-in kernels (1, 2) it covers the launch and the model, not the

@@ -568,6 +568,12 @@ int lsqed_ham_hv_fingerprint(lsqed_ham* h, uint64_t* fp) {
   const int grid = hv_launch_grid(h);
   if (grid <= 0) return fail(LSQED_ERR_HIP, "H.v plan setup failed");
   uint64_t f = fnv1a_val(k, fnv1a_val(grid, fnv1a_val(h->N, fnv1a_val(h->nup, fnv1a_val(h->build, 1469598103934665603ull)))));
+  if (k >= 2) {
+    // the order in which rows are dealt to blocks decides which gathers meet in cache
+    SplitTables t;
+    if (split_tables(h, t) != LSQED_OK) return fail(LSQED_ERR_HIP, "row-split plan setup failed");
+    f = fnv1a_val(t.order_hash, f);
+  }
   if (k == 3) {
     // the generated source holds the bonds, couplings, build mode and tuning
     f = fnv1a_val(jit_source_hash(h), fnv1a_val(jit_threads(h), f));

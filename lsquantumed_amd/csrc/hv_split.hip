@@ -67,6 +67,7 @@ struct SplitPlan {
   int ntasks_ord = 0;               // a multiple of 8 (padding items process no entries)
   uint32_t* d_rows_ord = nullptr;   // whole rows, same order and padding (row-staged kernel)
   int nrows_ord = 0;
+  uint64_t order_hash = 0;          // FNV-1a of the task and row orders (part of the H.v fingerprint)
   uint32_t* d_rows = nullptr;       // whole rows in h order, no padding (row-staged batched kernel)
   int nrows = 0;
   size_t binom_n = 0;
@@ -521,6 +522,7 @@ int split_plan_create(lsqed_ham* h, bool batch) {
   order_items(h, L, NH, ko, tasks_ord, rows_ord);
   sp->ntasks_ord = (int)tasks_ord.size();
   sp->nrows_ord = (int)rows_ord.size();
+  sp->order_hash = fnv1a(rows_ord.data(), rows_ord.size() * 4, fnv1a(tasks_ord.data(), tasks_ord.size() * 4));
   p.ntasks = sp->ntasks_ord;
   auto up = [&](void** dst, const void* src, size_t bytes) -> bool {
     if (hipMalloc(dst, std::max<size_t>(bytes, 4)) != hipSuccess) return false;
@@ -564,6 +566,7 @@ int split_tables(lsqed_ham* h, SplitTables& t, bool batch) {
   t.rows = sp->d_rows;
   t.nrows = sp->nrows;
   t.offU = sp->p.offU;
+  t.order_hash = sp->order_hash;
   return LSQED_OK;
 }
 

@@ -330,6 +330,155 @@ __global__ __launch_bounds__(kThreads) void k_dcgs_update(const double* __restri
   }
 }
 
+// Two-row forms of the delayed CGS2 passes and the restart: each thread takes
+// rows 2t and 2t+1 with 16-byte loads (columns 16-byte aligned: ld even),
+// halving the load instructions per byte; the arithmetic of each row is
+// that of the one-row kernels.
+template <int MAXM, bool HAS_Y>
+__global__ __launch_bounds__(kThreads) void k_dcgs_proj2(const double* __restrict__ V, int64_t ld, int j,
+                                                         const double* __restrict__ u, const double* __restrict__ y,
+                                                         int64_t n, double* __restrict__ part) {
+  constexpr int NQ = HAS_Y ? 2 * MAXM + 3 : MAXM + 1;
+  __shared__ double red[kThreads / 64][NQ];
+  int64_t a, b;
+  block_range(n / 2, a, b);
+  const double2* __restrict__ U2 = reinterpret_cast<const double2*>(u);
+  const double2* __restrict__ Y2 = reinterpret_cast<const double2*>(y);
+  double cu[MAXM], cy[MAXM];
+  double uu = 0.0, uy = 0.0, yy = 0.0;
+#pragma unroll
+  for (int i = 0; i < MAXM; ++i) cu[i] = cy[i] = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+    double2 vin[MAXM];
+#pragma unroll
+    for (int i = 0; i < MAXM; ++i)
+      vin[i] = i < j ? reinterpret_cast<const double2*>(V + (int64_t)i * ld)[r] : make_double2(0.0, 0.0);
+    const double2 ur = U2[r];
+    const double2 yr = HAS_Y ? Y2[r] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < MAXM; ++i) {
+      cu[i] += vin[i].x * ur.x;
+      cu[i] += vin[i].y * ur.y;
+      if (HAS_Y) {
+        cy[i] += vin[i].x * yr.x;
+        cy[i] += vin[i].y * yr.y;
+      }
+    }
+    uu += ur.x * ur.x;
+    uu += ur.y * ur.y;
+    if (HAS_Y) {
+      uy += ur.x * yr.x;
+      uy += ur.y * yr.y;
+      yy += yr.x * yr.x;
+      yy += yr.y * yr.y;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < MAXM; ++i) {
+    const double t = wsum(cu[i]);
+    if (lane == 0) red[wid][i] = t;
+    if (HAS_Y) {
+      const double t2 = wsum(cy[i]);
+      if (lane == 0) red[wid][MAXM + i] = t2;
+    }
+  }
+  {
+    const double t = wsum(uu);
+    if (lane == 0) red[wid][HAS_Y ? 2 * MAXM : MAXM] = t;
+  }
+  if (HAS_Y) {
+    const double t1 = wsum(uy), t2 = wsum(yy);
+    if (lane == 0) {
+      red[wid][2 * MAXM + 1] = t1;
+      red[wid][2 * MAXM + 2] = t2;
+    }
+  }
+  __syncthreads();
+  const int G = gridDim.x;
+  for (int q = threadIdx.x; q < NQ; q += blockDim.x) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w][q];
+    int out = -1;
+    if (q < MAXM) out = q < j ? q : -1;
+    else if (HAS_Y && q < 2 * MAXM) out = q - MAXM < j ? j + (q - MAXM) : -1;
+    else out = (HAS_Y ? 2 * j : j) + (q - (HAS_Y ? 2 * MAXM : MAXM));
+    if (out >= 0) part[(int64_t)out * G + blockIdx.x] = t;
+  }
+}
+
+template <int MAXM, bool HAS_Y>
+__global__ __launch_bounds__(kThreads) void k_dcgs_update2(const double* __restrict__ V, int64_t ld, int j,
+                                                           const double* __restrict__ coef, double inv_nu, double ya,
+                                                           double ub, double* __restrict__ u, double* __restrict__ y,
+                                                           int64_t n, double* __restrict__ part) {
+  __shared__ double cs[MAXM], zs[MAXM];
+  for (int i = threadIdx.x; i < MAXM; i += blockDim.x) {
+    cs[i] = i < j ? coef[i] : 0.0;
+    zs[i] = (HAS_Y && i < j) ? coef[j + i] : 0.0;
+  }
+  __syncthreads();
+  int64_t a, b;
+  block_range(n / 2, a, b);
+  double2* __restrict__ U2 = reinterpret_cast<double2*>(u);
+  double2* __restrict__ Y2 = reinterpret_cast<double2*>(y);
+  double acc = 0.0;
+  for (int64_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+    double2 vin[MAXM];
+#pragma unroll
+    for (int i = 0; i < MAXM; ++i)
+      vin[i] = i < j ? reinterpret_cast<const double2*>(V + (int64_t)i * ld)[r] : make_double2(0.0, 0.0);
+    const double2 ur = U2[r];
+    const double2 yr = HAS_Y ? Y2[r] : make_double2(0.0, 0.0);
+    double2 sc = make_double2(0.0, 0.0), sz = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < MAXM; ++i) {
+      sc.x += vin[i].x * cs[i];
+      sc.y += vin[i].y * cs[i];
+      if (HAS_Y) {
+        sz.x += vin[i].x * zs[i];
+        sz.y += vin[i].y * zs[i];
+      }
+    }
+    U2[r] = make_double2((ur.x - sc.x) * inv_nu, (ur.y - sc.y) * inv_nu);
+    if (HAS_Y) {
+      const double2 w = make_double2(ya * yr.x + ub * ur.x - sz.x, ya * yr.y + ub * ur.y - sz.y);
+      Y2[r] = w;
+      acc += w.x * w.x;
+      acc += w.y * w.y;
+    }
+  }
+  if (HAS_Y) {
+    const double t = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(kThreads) void k_restart2(double* __restrict__ V, int64_t ld, int m, int k,
+                                                       const double* __restrict__ S, int64_t n) {
+  __shared__ double Ss[MAXM * MAXM];
+  for (int i = threadIdx.x; i < m * k; i += blockDim.x) Ss[i] = S[i];
+  __syncthreads();
+  const int64_t n2 = n / 2;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n2; r += (int64_t)gridDim.x * blockDim.x) {
+    double2 vin[MAXM];
+#pragma unroll
+    for (int j = 0; j < MAXM; ++j)
+      vin[j] = (j < m) ? reinterpret_cast<const double2*>(V + (int64_t)j * ld)[r] : make_double2(0.0, 0.0);
+    for (int c = 0; c < k; ++c) {
+      double2 o = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < MAXM; ++j)
+        if (j < m) {
+          o.x += vin[j].x * Ss[c * m + j];
+          o.y += vin[j].y * Ss[c * m + j];
+        }
+      reinterpret_cast<double2*>(V + (int64_t)c * ld)[r] = o;
+    }
+  }
+}
+
 // In-place V[:, 0:k] = V[:, 0:m] S (S column-major m x k), row by row.
 template <int MAXM>
 __global__ __launch_bounds__(kThreads) void k_restart(double* __restrict__ V, int64_t ld, int m, int k,
@@ -734,6 +883,18 @@ int cgs2(Ctx& c, int m, double* w, std::vector<double>& h, double& nrm, bool* li
   return LSQED_OK;
 }
 
+// The two-row kernels apply when every column and vector is 16-byte aligned
+// (even leading dimension; LSQED_KS_TWO_ROWS=0 keeps the one-row kernels).
+static bool two_rows(const Ctx& c, const double* u, const double* y) {
+  static const bool on = [] {
+    const char* e = std::getenv("LSQED_KS_TWO_ROWS");
+    return !(e && e[0] == '0');
+  }();
+  const uintptr_t bits = reinterpret_cast<uintptr_t>(c.e->V) | reinterpret_cast<uintptr_t>(u) |
+                         reinterpret_cast<uintptr_t>(y);
+  return on && (c.D % 2 == 0) && (bits & 15) == 0;
+}
+
 // V[:, 0:k] = V[:, 0:m] S[:, 0:k] (S host row-major m x m, first k columns).
 int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
   std::vector<double> Sc((size_t)m * k);
@@ -741,7 +902,14 @@ int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
     for (int j = 0; j < m; ++j) Sc[(size_t)col * m + j] = S[(size_t)j * m + col];
   LSQED_HIP(hipMemcpyAsync(c.e->dsm, Sc.data(), Sc.size() * sizeof(double), hipMemcpyHostToDevice, c.s));
   const int g = stream_grid(c.D);
-  if (m <= 16)
+  if (two_rows(c, c.e->V, c.e->V)) {
+    if (m <= 16)
+      hipLaunchKernelGGL(k_restart2<16>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+    else if (m <= 32)
+      hipLaunchKernelGGL(k_restart2<32>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+    else
+      hipLaunchKernelGGL(k_restart2<64>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+  } else if (m <= 16)
     hipLaunchKernelGGL(k_restart<16>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
   else if (m <= 32)
     hipLaunchKernelGGL(k_restart<32>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
@@ -754,6 +922,15 @@ int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
 // Delayed CGS2 passes (k_dcgs_proj / k_dcgs_update) for j <= 32 basis columns.
 template <int MAXM>
 void launch_dcgs_proj(Ctx& c, int G, int j, const double* u, const double* y) {
+  if (two_rows(c, u, y)) {
+    if (y)
+      hipLaunchKernelGGL((k_dcgs_proj2<MAXM, true>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D,
+                         c.e->part);
+    else
+      hipLaunchKernelGGL((k_dcgs_proj2<MAXM, false>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D,
+                         c.e->part);
+    return;
+  }
   if (y)
     hipLaunchKernelGGL((k_dcgs_proj<MAXM, true>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D,
                        c.e->part);
@@ -778,6 +955,15 @@ int dcgs_project(Ctx& c, int j, const double* u, const double* y, std::vector<do
 template <int MAXM>
 void launch_dcgs_update(Ctx& c, int G, int j, const double* coef, double inv_nu, double ya, double ub, double* u,
                         double* y) {
+  if (two_rows(c, u, y)) {
+    if (y)
+      hipLaunchKernelGGL((k_dcgs_update2<MAXM, true>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, coef,
+                         inv_nu, ya, ub, u, y, c.D, c.e->part);
+    else
+      hipLaunchKernelGGL((k_dcgs_update2<MAXM, false>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, coef,
+                         inv_nu, ya, ub, u, y, c.D, c.e->part);
+    return;
+  }
   if (y)
     hipLaunchKernelGGL((k_dcgs_update<MAXM, true>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, coef, inv_nu,
                        ya, ub, u, y, c.D, c.e->part);

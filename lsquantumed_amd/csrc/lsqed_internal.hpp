@@ -149,6 +149,7 @@ struct SplitTables {
   const uint32_t* rows;      // whole rows in h order, no padding
   int nrows;
   const int* offU;
+  uint64_t order_hash;       // of the item and row orders
 };
 bool split_supported(const lsqed_ham* h);
 int split_low_bits(const lsqed_ham* h, bool batch = false);
