@@ -224,11 +224,28 @@ __global__ __launch_bounds__(kBThreads) void k_batch_update(const BatchProb* __r
     const int64_t off = probs[k].off;
     const double a = coef[3 * k], b = coef[3 * k + 1], c = coef[3 * k + 2];
     double acc = 0.0;
-    for (int r = task.row0 + threadIdx.x; r < task.row0 + task.nrows; r += kBThreads) {
-      const int64_t i = off + r;
-      const double v = a * y[i] + b * q[i] + c * p[i];
-      p[i] = v;
-      acc += v * v;
+    // four rows per lane per iteration, their 12 loads issued before any is used
+    const int end = task.row0 + task.nrows;
+    for (int r0 = task.row0; r0 < end; r0 += 4 * kBThreads) {
+      double vy[4], vq[4], vp[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + u * kBThreads + threadIdx.x;
+        if (r < end) {
+          vy[u] = y[off + r];
+          vq[u] = q[off + r];
+          vp[u] = p[off + r];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + u * kBThreads + threadIdx.x;
+        if (r < end) {
+          const double v = a * vy[u] + b * vq[u] + c * vp[u];
+          p[off + r] = v;
+          acc += v * v;
+        }
+      }
     }
     const double tot = block_sum(acc, red);
     if (threadIdx.x == 0) normpart[t] = tot;
