@@ -90,3 +90,32 @@ def test_hv_c128_4x8_rows(O, torch):
     a = torch.vdot(x, hz).item()
     b = torch.vdot(y, z).item()
     assert abs(a - b) <= 1e-10 * (abs(a) + abs(b)), (a, b)
+
+
+@pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
+def test_hv_c128_36_sites_wide_ranks(O, torch, kernel):
+    """The complex product on a 36-site lattice (64-bit states and row bases:
+    the wide forms of both kernels), 6x6 J1-J2 in the sector with 10 up spins
+    (D = 254,186,856): x = (1 + 2i) f with f the oracle's formula vector of
+    the rank; rows across the sector equal (1 + 2i) times the oracle's rows,
+    and each part equals the real product of that part bit for bit."""
+    from test_gpu_hamiltonian import formula_vector
+    say("hv c128 36 sites", kernel)
+    h = make("square", 36, 10 - 18, 1.0, 1.0, 0.5, 0.5, 6, 6)
+    h.set_hv_kernel(kernel)
+    D = h.hamilt_size()
+    assert D == 254186856
+    f = formula_vector(O, torch, D)
+    x = torch.complex(f, 2.0 * f)
+    y = h.matvec_c128(x)
+    assert torch.equal(y.real, h.matvec(x.real.contiguous()))
+    assert torch.equal(y.imag, h.matvec(x.imag.contiguous()))
+    m = oracle_model(O, "square", 36, 10 - 18, 1.0, 1.0, 0.5, 0.5, 6, 6)
+    for row0 in (0, D // 3, D - 3000):
+        yref = m.hv_rows_formula(row0, 3000)
+        got = y[row0:row0 + 3000].cpu().numpy()
+        scale = np.abs(yref).max()
+        assert np.max(np.abs(got.real - yref)) <= 1e-12 * scale, row0
+        assert np.max(np.abs(got.imag - 2.0 * yref)) <= 2e-12 * scale, row0
+    del f, x, y
+    torch.cuda.empty_cache()
