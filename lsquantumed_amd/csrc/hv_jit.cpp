@@ -42,6 +42,17 @@ static const char* kJitBody = R"JIT(
 #ifndef NT_STAGE
 #define NT_STAGE 0
 #endif
+// LOWTAB (with ROWSTAGE): the low bonds of an entry come from a host-built
+// record per low pattern l -- for each flipping low bond (LOWFK order) the
+// partner's index in the row, or RMAX (a zero entry of the staged row) when
+// its sites are parallel, then l and the antiparallel counts of both kinds --
+// instead of popcounts and rank-table lookups per bond
+#ifndef LOWTAB
+#define LOWTAB 0
+#endif
+#ifndef PREFETCH
+#define PREFETCH 0
+#endif
 // ROWSTAGE: blocks of ROW_THREADS threads take whole rows, staging each
 // row's x in LDS, so that the low bonds (targets inside the row) read LDS
 // instead of issuing global gathers
@@ -108,6 +119,7 @@ struct JitArgs {
   const unsigned int* tasks;
   int ntasks;
   int offU[17];
+  const uint4* lowtab;     // LOWTAB: RECQ x 16 bytes per low pattern, in U's order
 #if BATCH
   const long long* poff;   // per problem: offset of its entries in x and y
   const double* pcoup;     // per problem: J1/2, J2/2, D1/4, D2/4
@@ -139,9 +151,12 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   __shared__ double s_dish[NBUF];
   __shared__ double s_red[kThreads / 64];
 #if ROWSTAGE
-  __shared__ XV s_x[RMAX];
+  __shared__ XV s_x[RMAX + LOWTAB];
 #endif
   const int tid = threadIdx.x;
+#if ROWSTAGE && LOWTAB
+  if (tid == 0) s_x[RMAX] = xzero();  // rows have at most RMAX entries: never overwritten
+#endif
   for (int i = tid; i < 256; i += kThreads) s_RL0[i] = a.RL0[i];
   for (int i = tid; i < 9 * W1; i += kThreads) s_RL1[i] = a.RL1[i];
 #if WIDE
@@ -332,6 +347,18 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #pragma unroll
     for (int q = 0; q < NH; ++q) cb[q] = rfl(s_cbase[pb][q]);
     const unsigned short* Uk = a.U + a.offU[kl];
+#if ROWSTAGE && PREFETCH
+    // the pattern (or record) of a lane's next entry is loaded one entry
+    // ahead, so the cross gathers, which need it, do not wait for its load
+#if LOWTAB
+    const uint4* __restrict__ recb = a.lowtab + (unsigned int)a.offU[kl] * RECQ;
+    uint4 rq[RECQ];
+#pragma unroll
+    for (int q = 0; q < RECQ; ++q) rq[q] = tid < R ? recb[(unsigned int)tid * RECQ + q] : make_uint4(0u, 0u, 0u, 0u);
+#else
+    unsigned int l_nx = tid < R ? (unsigned int)Uk[tid] : 0u;
+#endif
+#endif
 #pragma unroll 1
 #if ROWSTAGE
     for (int k = 0; k < kStage; ++k) {
@@ -356,7 +383,34 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         for (int u = 0; u < kG; ++u)
           if (b0 + u < nact) xmadd(acc, s_hcoef[pb][b0 + u], v[u]);
       }
+#if LOWTAB
+      unsigned int rw[4 * RECQ];
+      {
+#if ROWSTAGE && PREFETCH
+        const bool more = i + kThreads < R;
+#pragma unroll
+        for (int q = 0; q < RECQ; ++q) {
+          const uint4 w = rq[q];
+          if (more) rq[q] = recb[(unsigned int)(i + kThreads) * RECQ + q];
+#else
+        const uint4* __restrict__ rec = a.lowtab + (unsigned int)(a.offU[kl] + i) * RECQ;
+#pragma unroll
+        for (int q = 0; q < RECQ; ++q) {
+          const uint4 w = rec[q];
+#endif
+          rw[4 * q] = w.x;
+          rw[4 * q + 1] = w.y;
+          rw[4 * q + 2] = w.z;
+          rw[4 * q + 3] = w.w;
+        }
+      }
+      const unsigned int l = rw[LWORD] & 0xffffu;
+#elif ROWSTAGE && PREFETCH
+      const unsigned int l = l_nx;
+      if (i + kThreads < R) l_nx = Uk[i + kThreads];
+#else
       const unsigned int l = Uk[i];
+#endif
       const unsigned int v0 = l & 255u;
       const int k0 = __builtin_popcount(v0);
       const unsigned int t1 = s_RL1[k0 * W1 + (l >> 8)];
@@ -402,6 +456,19 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
+#if LOWTAB
+      // low bonds: inside row h, partner entries from the record (a parallel
+      // pair reads the zero entry, so every flipping bond is one LDS read and
+      // one FMA, summed in bond order as below)
+#pragma unroll
+      for (int b = 0; b < NLOWF; ++b) {
+        const unsigned int w = rw[b >> 1];
+        const unsigned int j = (b & 1) ? (w >> 16) : (w & 0xffffu);
+        xmadd(acc, jh_[LOWFK[b]], s_x[j]);
+      }
+      cc0 += (int)((rw[LWORD] >> 16) & 0xffu);
+      cc1 += (int)(rw[LWORD] >> 24);
+#else
       // low bonds: inside row h, entry RL(l ^ m)
 #pragma unroll
       for (int b0 = 0; b0 < NLOW; b0 += kG) {
@@ -445,18 +512,24 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
+#endif
       const int c0 = NB0 - 2 * (cnt0 + cc0);
       const int c1 = NB1 - 2 * (cnt1 + cc1);
       double d = 0.0;
-      if (DIAG_MODE == 0) {
-        if (DZ0 && c0 != 0) d = dq0_ * (double)c0;
-        if (DZ1 && c1 != 0) d = d + dq1_ * (double)c1;
-      } else if (DIAG_MODE == 1) {
-        double dis = 0.0;
+      {
+        // no contraction: one rounding per operation, whichever kernel
+        // variant (real, complex, batched) is compiled
+#pragma clang fp contract(off)
+        if (DIAG_MODE == 0) {
+          if (DZ0 && c0 != 0) d = dq0_ * (double)c0;
+          if (DZ1 && c1 != 0) d = d + dq1_ * (double)c1;
+        } else if (DIAG_MODE == 1) {
+          double dis = 0.0;
 #pragma unroll
-        for (int q = 0; q < L; ++q) dis = ((l >> q) & 1u) ? dis + s_f[q] : dis - s_f[q];
-        dis += s_dish[pb];
-        d = (dq0_ * (double)c0 + dq1_ * (double)c1) + 0.5 * dis;
+          for (int q = 0; q < L; ++q) dis = ((l >> q) & 1u) ? dis + s_f[q] : dis - s_f[q];
+          dis += s_dish[pb];
+          d = (dq0_ * (double)c0 + dq1_ * (double)c1) + 0.5 * dis;
+        }
       }
 #if ROWSTAGE
       const XV xr = s_x[i];
@@ -536,6 +609,7 @@ struct JitArgs {
   const uint32_t* tasks;
   int ntasks;
   int offU[17];
+  const void* lowtab;
 };
 
 // Argument block of the batch variant (same leading layout).
@@ -556,6 +630,7 @@ struct JitModule {
   bool rowstage = false;  // one block per row (rows_ord), else 256 threads per item
   int threads = 256;      // block size
   uint64_t src_hash = 0;  // FNV-1a of the generated source
+  void* d_lowtab = nullptr;  // LOWTAB records on the module's device (kept for the process)
   std::string log;
 };
 
@@ -600,10 +675,59 @@ static std::string dbl(double v) {
   return s;
 }
 
+// Low-bond records (LOWTAB) for row-staged kernels with L <= 14: on for the
+// batched kernel, whose chains of N = 18-22 spend most VALU issue on the
+// per-bond rank arithmetic of their 25 low bonds (VALU 66% busy on the N = 22
+// launch); LSQED_JIT_LOWTAB=0/1 overrides for both kernels.  Off for single
+// models: the 4x8 H.v took 38.92 ms with records against 37.2 ms without, 6x6
+// 726 against 686 ms (the extra record loads on an HBM-bound kernel).
+static bool jit_lowtab(bool batch, int L) {
+  if (L > 14) return false;
+  const char* e = getenv("LSQED_JIT_LOWTAB");
+  return e ? atoi(e) == 1 : batch;
+}
+
+// Records of the LOWTAB kernel, one per low pattern l in U's order (popcount
+// class, then numeric = colex order), `recq` x 4 words each: words
+// 0..(nf+1)/2-1 hold the row indices of l ^ m for the flipping low bonds m
+// (two 16-bit halves per word, bond order), or `zslot` when the bond's sites
+// are parallel; word (nf+1)/2 holds l | cc0 << 16 | cc1 << 24, the counts of
+// antiparallel low bonds of kind 0 and 1 (every low bond, flipping or not).
+static std::vector<uint32_t> lowtab_records(int L, const std::vector<unsigned>& lowm, const std::vector<int>& lowk,
+                                            const int flip[2], int recq, unsigned zslot) {
+  std::vector<int> f;
+  for (size_t b = 0; b < lowm.size(); ++b)
+    if (flip[lowk[b]]) f.push_back((int)b);
+  const int lword = ((int)f.size() + 1) / 2;
+  auto rank = [&](unsigned v) {
+    uint32_t r = 0;
+    int j = 0;
+    for (int q = 0; q < L; ++q)
+      if ((v >> q) & 1u) r += (uint32_t)binom_u64(q, ++j);
+    return r;
+  };
+  std::vector<uint32_t> rec;
+  for (int k = 0; k <= L; ++k)
+    for (unsigned v = 0; v < (1u << L); ++v) {
+      if (__builtin_popcount(v) != k) continue;
+      std::vector<uint32_t> w((size_t)4 * recq, 0);
+      for (size_t t = 0; t < f.size(); ++t) {
+        const unsigned m = lowm[f[t]];
+        const uint32_t j = __builtin_popcount(v & m) == 1 ? rank(v ^ m) : zslot;
+        w[t / 2] |= (t & 1) ? j << 16 : j;
+      }
+      uint32_t cc[2] = {0, 0};
+      for (size_t b = 0; b < lowm.size(); ++b) cc[lowk[b]] += __builtin_popcount(v & lowm[b]) == 1;
+      w[lword] = v | cc[0] << 16 | cc[1] << 24;
+      rec.insert(rec.end(), w.begin(), w.end());
+    }
+  return rec;
+}
+
 // Source of the kernel specialised to the bond classes and couplings of `h`
 // (row split with L = split_low_bits(h, batch) low bits, see hv_split.hip).  Fields stay a
 // run-time array, so disorder realisations share one compiled kernel.
-std::string jit_source(const lsqed_ham* h, bool batch, bool cplx) {
+std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<uint32_t>* lowtab) {
   const int N = h->N, nup = h->nup, L = split_low_bits(h, batch), NH = N - L;
   std::vector<unsigned> lowm, him;
   std::vector<int> lowk, lowt, crq, crp, crk, hik;
@@ -655,13 +779,33 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx) {
     if (atoi(w) == 1) o << "#define NT_STORE 1\n";
   if (const char* w = getenv("LSQED_JIT_NT_STAGE"))
     if (atoi(w) == 1) o << "#define NT_STAGE 1\n";
-  if (jit_rowstage())
-    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L)
-      << "\nconstexpr int RMAX = " << binom_u64(L, L / 2) << ";\n";
+  // The next entry's pattern loaded one entry ahead (row-staged kernels;
+  // LSQED_JIT_PREFETCH=0 turns it off): 4x8 H.v 37.13 -> 36.82-36.87 ms, the
+  // batched disorder solve 938 -> 957 realisations/s (980 with LOWTAB records,
+  // 959 with records alone).
+  {
+    const char* w = getenv("LSQED_JIT_PREFETCH");
+    if (!(w && atoi(w) == 0)) o << "#define PREFETCH 1\n";
+  }
+  const RowParams rp = make_row_params(h);
+  if (jit_rowstage()) {
+    const uint64_t rmax = binom_u64(L, L / 2);
+    o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L) << "\nconstexpr int RMAX = " << rmax
+      << ";\n";
+    if (jit_lowtab(batch, L)) {
+      std::vector<int> lowfk;
+      for (size_t b = 0; b < lowm.size(); ++b)
+        if (rp.flip[lowk[b]]) lowfk.push_back(lowk[b]);
+      const int nf = (int)lowfk.size(), lword = (nf + 1) / 2, recq = (lword + 1 + 3) / 4;
+      o << "#define LOWTAB 1\nconstexpr int NLOWF = " << nf << ", LWORD = " << lword << ", RECQ = " << recq
+        << ";\n";
+      emit_array(o, "LOWFK", lowfk, "int");
+      if (lowtab) *lowtab = lowtab_records(L, lowm, lowk, rp.flip, recq, (unsigned)rmax);
+    }
+  }
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
     << ", W1 = " << (1 << std::max(L - 8, 0)) << ";\n";
   o << "constexpr int NB0 = " << h->bi[0].size() << ", NB1 = " << h->bi[1].size() << ";\n";
-  const RowParams rp = make_row_params(h);
   // the batch variant reads couplings per problem at run time
   o << "constexpr double JH0 = " << dbl(batch ? 0.0 : rp.jh[0]) << ", JH1 = " << dbl(batch ? 0.0 : rp.jh[1])
     << ", DQ0 = " << dbl(batch ? 0.0 : rp.dq[0]) << ", DQ1 = " << dbl(batch ? 0.0 : rp.dq[1]) << ";\n";
@@ -731,21 +875,34 @@ bool jit_enabled() {
   return !(e && e[0] == '0');
 }
 
-int jit_prepare(lsqed_ham* h) {
-  if (h->jit) return LSQED_OK;
-  const std::string src = jit_source(h, false, false);
-  std::shared_ptr<JitModule> m;
-  {
-    std::lock_guard<std::mutex> lk(g_jit_mu);
-    const auto keyd = std::make_pair(h->device, src);
-    auto it = g_jit_cache.find(keyd);
-    if (it != g_jit_cache.end()) {
-      m = it->second;
-    } else {
-      m = jit_compile(src, false);
-      g_jit_cache[keyd] = m;
+// The module of (h's device, generated source), compiled on first use; a
+// LOWTAB module's records are built and uploaded once, with the module, to the
+// current device (the handle's, as for the module itself).
+static std::shared_ptr<JitModule> jit_module(const lsqed_ham* h, bool batch, bool cplx, std::string* key) {
+  const std::string src = jit_source(h, batch, cplx, nullptr);
+  if (key) *key = src;
+  std::lock_guard<std::mutex> lk(g_jit_mu);
+  const auto keyd = std::make_pair(h->device, src);
+  auto it = g_jit_cache.find(keyd);
+  if (it != g_jit_cache.end()) return it->second;
+  std::shared_ptr<JitModule> m = jit_compile(src, batch);
+  std::vector<uint32_t> tab;
+  if (m->ok && src.find("#define LOWTAB 1") != std::string::npos) {
+    (void)jit_source(h, batch, cplx, &tab);
+    const size_t bytes = tab.size() * sizeof(uint32_t);
+    if (tab.empty() || hipMalloc(&m->d_lowtab, bytes) != hipSuccess ||
+        hipMemcpy(m->d_lowtab, tab.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      m->ok = false;
+      m->log += "\nupload of the low-bond records failed";
     }
   }
+  g_jit_cache[keyd] = m;
+  return m;
+}
+
+int jit_prepare(lsqed_ham* h) {
+  if (h->jit) return LSQED_OK;
+  std::shared_ptr<JitModule> m = jit_module(h, false, false, nullptr);
   if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the specialised H.v failed: " + m->log);
   h->jit = m.get();
   return LSQED_OK;
@@ -787,19 +944,7 @@ int jit_threads(const lsqed_ham* h) { return h->jit ? ((const JitModule*)h->jit)
 // The complex128 module of the handle (same bond structure and couplings).
 static int jit_prepare_c128(lsqed_ham* h) {
   if (h->jit_c) return LSQED_OK;
-  const std::string src = jit_source(h, false, true);
-  std::shared_ptr<JitModule> m;
-  {
-    std::lock_guard<std::mutex> lk(g_jit_mu);
-    const auto keyd = std::make_pair(h->device, src);
-    auto it = g_jit_cache.find(keyd);
-    if (it != g_jit_cache.end()) {
-      m = it->second;
-    } else {
-      m = jit_compile(src, false);
-      g_jit_cache[keyd] = m;
-    }
-  }
+  std::shared_ptr<JitModule> m = jit_module(h, false, true, nullptr);
   if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the complex specialised H.v failed: " + m->log);
   h->jit_c = m.get();
   return LSQED_OK;
@@ -827,6 +972,7 @@ int launch_hv_jit_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s) 
   a.tasks = m->rowstage ? t.rows_ord : t.tasks_ord;
   a.ntasks = m->rowstage ? t.nrows_ord : t.ntasks_ord;
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
+  a.lowtab = m->d_lowtab;
   void* args[] = {&a};
   LSQED_HIP(hipModuleLaunchKernel(m->fn_nodot, (unsigned)grid, 1, 1, m->threads, 1, 1, 0, s, args, nullptr));
   return LSQED_OK;
@@ -852,6 +998,7 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
   a.tasks = m->rowstage ? t.rows_ord : t.tasks_ord;
   a.ntasks = m->rowstage ? t.nrows_ord : t.ntasks_ord;
   for (int k = 0; k < 17; ++k) a.offU[k] = t.offU[k];
+  a.lowtab = m->d_lowtab;
   void* args[] = {&a};
   LSQED_HIP(hipModuleLaunchKernel(dot_part ? m->fn_dot : m->fn_nodot, (unsigned)grid, 1, 1, m->threads,
                                   1, 1, 0, s, args, nullptr));
@@ -862,20 +1009,7 @@ int launch_hv_jit(lsqed_ham* h, const double* x, double* y, double* dot_part, in
 // fields per problem at run time).  *key receives the generated source, which
 // identifies the group of problems that can share the kernel.
 int batch_jit_prepare(lsqed_ham* h, void** module, std::string* key) {
-  const std::string src = jit_source(h, true, false);
-  if (key) *key = src;
-  std::shared_ptr<JitModule> m;
-  {
-    std::lock_guard<std::mutex> lk(g_jit_mu);
-    const auto keyd = std::make_pair(h->device, src);
-    auto it = g_jit_cache.find(keyd);
-    if (it != g_jit_cache.end()) {
-      m = it->second;
-    } else {
-      m = jit_compile(src, true);
-      g_jit_cache[keyd] = m;
-    }
-  }
+  std::shared_ptr<JitModule> m = jit_module(h, true, false, key);
   if (!m->ok) return fail(LSQED_ERR_HIP, "hipRTC compile of the batched H.v failed: " + m->log);
   *module = m.get();
   return LSQED_OK;
@@ -909,6 +1043,7 @@ int launch_hv_batch_jit(void* module, lsqed_ham* h0, const BatchJitLaunch& L, in
   a.base.tasks = m->rowstage ? t.rows : t.tasks;
   a.base.ntasks = m->rowstage ? t.nrows : t.ntasks;
   for (int k = 0; k < 17; ++k) a.base.offU[k] = t.offU[k];
+  a.base.lowtab = m->d_lowtab;
   a.poff = L.poff;
   a.pcoup = L.pcoup;
   a.pfields = L.pfields;

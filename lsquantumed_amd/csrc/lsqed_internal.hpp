@@ -169,6 +169,10 @@ int launch_hv_jit_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s);
 int launch_hv_c128(lsqed_ham* h, const double* x, double* y, hipStream_t s);
 int jit_grid(lsqed_ham* h);
 uint64_t jit_source_hash(const lsqed_ham* h);  // of the prepared module's generated source (0: none)
+// Generated source of the specialised kernel (batch: the per-problem-coupling
+// variant; cplx: complex128 entries).  With lowtab non-null and the source
+// using low-bond records (LOWTAB), *lowtab receives them (hv_jit.cpp).
+std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<uint32_t>* lowtab = nullptr);
 int jit_threads(const lsqed_ham* h);
 
 // Batch variant of the specialised kernel (hv_jit.cpp): problems sharing a

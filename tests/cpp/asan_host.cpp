@@ -4,17 +4,17 @@
 // sector shape (single-model and batched), the bond lists and their argument
 // errors (src/lattice.cpp:67-68, :153-154), the disorder RNG
 // (src/tools/random_disorder.cpp:33-59), sector sizes, and the clean-up of
-// lsqed_ham_create when no device is present.  Exits non-zero on a wrong
+// lsqed_ham_create when no device is present, and the low-bond records of
+// the LOWTAB kernels against brute force.  Exits non-zero on a wrong
 // result; the sanitizer aborts on a memory error or leak.
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "lsqed_internal.hpp"
 
-namespace lsqed {
-std::string jit_source(const lsqed_ham* h, bool batch, bool cplx);
-}
 
 static int failures = 0;
 #define EXPECT(c)                                                   \
@@ -62,7 +62,72 @@ static int generate(int lat, int N, int lx, int ly) {
   return n;
 }
 
+// Low-bond records of a LOWTAB source against brute force: patterns of each
+// popcount class listed in numeric order, a partner's index looked up in that
+// list (no rank formula), the antiparallel counts recounted bond by bond.
+static int check_records(int lat, int N, int lx, int ly, bool batch, double J2) {
+  lsqed_ham h;
+  h.N = N;
+  h.nup = N / 2;
+  h.J[0] = 1.0;
+  h.J[1] = J2;
+  h.Dz[0] = 1.0;
+  h.Dz[1] = 0.3;
+  EXPECT(bonds(lat, N, lx, ly, h));
+  std::vector<uint32_t> rec;
+  const std::string s = lsqed::jit_source(&h, batch, false, &rec);
+  EXPECT(s.find("#define LOWTAB 1") != std::string::npos);
+  const int L = lsqed::split_low_bits(&h, batch);
+  std::vector<unsigned> m;
+  std::vector<int> k, flipping;
+  for (int kind = 0; kind < 2; ++kind)
+    for (size_t b = 0; b < h.bi[kind].size(); ++b) {
+      const int a = std::min(h.bi[kind][b], h.bj[kind][b]), c = std::max(h.bi[kind][b], h.bj[kind][b]);
+      if (a == c || c >= L) continue;
+      m.push_back((1u << a) | (1u << c));
+      k.push_back(kind);
+      if (h.J[kind] != 0.0) flipping.push_back((int)m.size() - 1);
+    }
+  const int nf = (int)flipping.size(), lword = (nf + 1) / 2, recq = (lword + 4) / 4;
+  std::vector<std::vector<unsigned>> cls(L + 1);
+  std::vector<int> pos(1u << L);
+  for (unsigned v = 0; v < (1u << L); ++v) {
+    const int c = __builtin_popcount(v);
+    pos[v] = (int)cls[c].size();
+    cls[c].push_back(v);
+  }
+  EXPECT(rec.size() == ((size_t)1 << L) * 4 * recq);
+  if (rec.size() != ((size_t)1 << L) * 4 * recq) return 0;
+  int rmax = 0;
+  for (const auto& c : cls) rmax = std::max<int>(rmax, (int)c.size());
+  size_t g = 0, bad = 0;
+  for (int c = 0; c <= L; ++c)
+    for (unsigned v : cls[c]) {
+      const uint32_t* w = &rec[g * 4 * recq];
+      bad += (w[lword] & 0xffffu) != v;
+      int cc[2] = {0, 0};
+      for (size_t b = 0; b < m.size(); ++b) cc[k[b]] += __builtin_popcount(v & m[b]) == 1;
+      bad += (int)((w[lword] >> 16) & 0xffu) != cc[0] || (int)(w[lword] >> 24) != cc[1];
+      for (int t = 0; t < nf; ++t) {
+        const unsigned mb = m[flipping[t]];
+        const int want = __builtin_popcount(v & mb) == 1 ? pos[v ^ mb] : rmax;
+        const int got = (int)((t & 1) ? (w[t / 2] >> 16) : (w[t / 2] & 0xffffu));
+        bad += got != want;
+      }
+      ++g;
+    }
+  EXPECT(bad == 0);
+  return 1;
+}
+
 int main() {
+  int nrec = check_records(LSQED_CHAIN1D, 22, 0, 0, true, 0.2);
+  nrec += check_records(LSQED_CHAIN1D, 18, 0, 0, true, 0.0);  // NNN bonds do not flip, still counted
+  setenv("LSQED_JIT_LOWTAB", "1", 1);
+  nrec += check_records(LSQED_SQUARE2D, 32, 4, 8, false, 0.5);
+  nrec += check_records(LSQED_HONEYCOMB2D, 24, 4, 6, false, 0.4);
+  unsetenv("LSQED_JIT_LOWTAB");
+  EXPECT(nrec == 4);
   int nsrc = 0;
   for (int N = 6; N <= 40; N += 2) nsrc += generate(LSQED_CHAIN1D, N, 0, 0);
   const int sq[][2] = {{2, 4}, {3, 4}, {4, 4}, {4, 5}, {4, 6}, {2, 12}, {4, 7}, {4, 8}, {8, 4}, {6, 6}, {4, 10}, {6, 1}};
