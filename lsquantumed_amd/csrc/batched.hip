@@ -28,6 +28,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -35,6 +37,7 @@
 #include <vector>
 
 #include "lsqed_internal.hpp"
+#include "replay_schedule.hpp"
 #include "sector_device.hpp"
 
 namespace lsqed {
@@ -244,6 +247,58 @@ __global__ __launch_bounds__(kBThreads) void k_batch_update(const BatchProb* __r
           const double v = a * vy[u] + b * vq[u] + c * vp[u];
           p[off + r] = v;
           acc += v * v;
+        }
+      }
+    }
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) normpart[t] = tot;
+  }
+}
+
+// The same update on pairs of rows (16-byte loads and stores), for batches
+// whose problem offsets and task lengths are all even: four pairs per lane
+// per iteration, their 12 loads issued before any is used; the task's share of
+// ||p||^2 sums the pairs' squares in lane order.
+__global__ __launch_bounds__(kBThreads) void k_batch_update2(const BatchProb* __restrict__ probs,
+                                                             const BTask* __restrict__ tasks, int ntasks,
+                                                             const int* __restrict__ active,
+                                                             const double* __restrict__ coef,
+                                                             const double* __restrict__ y,
+                                                             const double* __restrict__ q, double* __restrict__ p,
+                                                             double* __restrict__ normpart) {
+  __shared__ double red[kBThreads / 64];
+  for (int t = first_task(); t < ntasks; t += gridDim.x) {
+    const BTask task = tasks[t];
+    const int k = task.prob;
+    if (!active[k]) continue;
+    const int64_t off = probs[k].off;
+    const double a = coef[3 * k], b = coef[3 * k + 1], c = coef[3 * k + 2];
+    const double2* __restrict__ y2 = (const double2*)(y + off);
+    const double2* __restrict__ q2 = (const double2*)(q + off);
+    double2* __restrict__ p2 = (double2*)(p + off);
+    double acc = 0.0;
+    const int end = (task.row0 + task.nrows) >> 1;
+    for (int i0 = task.row0 >> 1; i0 < end; i0 += 4 * kBThreads) {
+      double2 vy[4], vq[4], vp[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBThreads + threadIdx.x;
+        if (i < end) {
+          vy[u] = y2[i];
+          vq[u] = q2[i];
+          vp[u] = p2[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBThreads + threadIdx.x;
+        if (i < end) {
+          double2 v;
+          v.x = a * vy[u].x + b * vq[u].x + c * vp[u].x;
+          v.y = a * vy[u].y + b * vq[u].y + c * vp[u].y;
+          p2[i] = v;
+          acc += v.x * v.x;
+          acc += v.y * v.y;
         }
       }
     }
@@ -486,6 +541,7 @@ struct lsqed_batch {
   std::vector<int64_t> off, dims;
   Shape shape;
   int grid = 0, ntasks = 0;
+  bool pairs = false;  // every problem offset and task length even: the update moves double2
   size_t smem = 0;
   int cus = 256;
   // device
@@ -662,6 +718,10 @@ int upload_problems(lsqed_batch* b) {
   LSQED_HIP(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice));
   LSQED_HIP(hipMemcpy(b->d_probs, probs.data(), probs.size() * sizeof(BatchProb), hipMemcpyHostToDevice));
   b->grid = grid_for(b, b->ntasks);
+  const char* pe = std::getenv("LSQED_BATCH_PAIRS");  // 0: the one-row update
+  b->pairs = !(pe && std::atoi(pe) == 0);
+  for (const BTask& t : tasks) b->pairs = b->pairs && (t.row0 % 2 == 0) && (t.nrows % 2 == 0);
+  for (int k = 0; k < b->nprob; ++k) b->pairs = b->pairs && (b->off[k] % 2 == 0);
   return LSQED_OK;
 }
 
@@ -746,8 +806,9 @@ int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, h
   hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, hv_partials(b), sc(b, S_QY), 1);
   hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_QY), sc(b, S_SQ),
                      sc(b, S_SP), sc(b, S_BETA), hist_alpha(b, j), sc(b, S_COEF));
-  hipLaunchKernelGGL(k_batch_update, dim3(b->grid), dim3(kBThreads), 0, s, b->d_probs, b->d_tasks, b->ntasks,
-                     b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part);
+  const bool aligned = (((uintptr_t)q | (uintptr_t)p | (uintptr_t)b->d_y) & 15u) == 0;
+  hipLaunchKernelGGL(b->pairs && aligned ? k_batch_update2 : k_batch_update, dim3(b->grid), dim3(kBThreads), 0, s,
+                     b->d_probs, b->d_tasks, b->ntasks, b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part);
   hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN), 0);
   hipLaunchKernelGGL(k_batch_beta, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_NN), sc(b, S_SQ),
                      sc(b, S_SP), sc(b, S_BETA), hist_beta(b, j));
@@ -909,7 +970,8 @@ int time_shape(lsqed_batch* b, float* ms) {
 // Autotune, timed on the batch itself and cached per batch shape:
 //   chunk-table kernel: rows per task x blocks per CU at one lane per state,
 //   then 2 and 4 lanes per state at the best tile (or the fixed lane count);
-//   specialised row split: blocks per CU.
+//   specialised row split: blocks per CU, then rows per task (which set the
+//   update's tasks there).
 int autotune(lsqed_batch* b) {
   const std::string key = shape_key(b);
   {
@@ -954,9 +1016,17 @@ int autotune(lsqed_batch* b) {
         if ((rc = trial(Shape{0, base.rows, lps, base.bpc}))) return rc;
     }
   }
-  if (try_spec)
+  if (try_spec) {
     for (int bpc : {2, 4, 8, 32, 1 << 20})  // 1 << 20: one work item per block
       if ((rc = trial(Shape{1, rows_best, 1, bpc}))) return rc;
+    // rows per task still set the update's, the partial sums' and the start
+    // vector's tasks under the specialised H.v (whose own tasks are rows)
+    if (best.kernel == 1) {
+      const Shape sb = best;
+      for (int rows : {2048, 4096, 8192})
+        if (rows != sb.rows && (rc = trial(Shape{1, rows, 1, sb.bpc}))) return rc;
+    }
+  }
   b->shape = best;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = best;
@@ -1113,62 +1183,44 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream) {
   b->steps.assign(np, 0);
   b->steps_vec.assign(np, 0);
   b->conv.assign(np, 0);
-  // Activity of each model in each enqueued replay is decided on the host:
-  //   done[k]      converged (or exhausted / out of steps): inactive from the
-  //                next enqueued replay on; the steps of the replay already in
-  //                flight are not used.
-  //   paused[k]    predicted to converge within the replay already in flight
-  //                (residual ratio of the last two replays): inactive from the
-  //                next enqueued replay; if the in-flight replay does not
-  //                confirm convergence it becomes active again.  A paused
-  //                model is untouched by every kernel, so pausing does not
-  //                change its Lanczos sequence, only saves the steps of a
-  //                replay it would not need.
-  std::vector<char> done(np, 0), paused(np, 0), cur(np, 1);
-  std::vector<double> prev_res(np, 0.0);
-  std::vector<std::vector<char>> act_at;  // activity vector of every enqueued replay
+  // Which replays run which model is decided on the host (replay_schedule.hpp).
+  ReplaySchedule sch(np);
   const size_t HS = (size_t)2 * kSteps * np;  // doubles per history buffer
   int slot = 0;                               // ring slot of the next activity upload
-  bool dirty = false;                         // cur differs from the device flags
   std::vector<double> ritz;
   auto enqueue = [&]() -> int {
-    const int r = (int)act_at.size();
-    if (dirty) {
+    const int r = (int)sch.act_at.size();
+    if (sch.dirty) {
       int* act = b->h_active + (size_t)(slot % b->ring) * np;
       ++slot;
-      for (int k = 0; k < np; ++k) act[k] = cur[k];
+      for (int k = 0; k < np; ++k) act[k] = sch.cur[k];
       LSQED_HIP(hipMemcpyAsync(b->d_active, act, np * sizeof(int), hipMemcpyHostToDevice, b->gs));
-      dirty = false;
     }
-    act_at.push_back(cur);
+    sch.enqueued();
     const int rc2 = run_replay(b, 0);
     if (rc2) return rc2;
     LSQED_HIP(hipMemcpyAsync(b->h_hist + (r & 1) * HS, b->d_hist, HS * sizeof(double), hipMemcpyDeviceToHost, b->gs));
     LSQED_HIP(hipEventRecord(b->ev[r & 1], b->gs));
     return LSQED_OK;
   };
-  auto any_active = [&]() {
-    for (int k = 0; k < np; ++k)
-      if (cur[k]) return true;
-    return false;
-  };
-  int done_r = 0, ndone = 0;
   if ((rc = enqueue())) return rc;
-  while (ndone < np) {
-    // keep one replay in flight beyond the one being processed (a model
-    // resumed after a failed prediction may need a replay when none is left)
-    if ((int)act_at.size() <= done_r + 1 && any_active())
+  while (!sch.finished()) {
+    while (sch.want_replay())
       if ((rc = enqueue())) return rc;
-    if ((int)act_at.size() <= done_r) {
+    if (sch.stalled()) {
       (void)hipStreamSynchronize(b->gs);
-      return fail(LSQED_ERR_STATE, "batched Lanczos: no replay in flight");
+      int npaused = 0;
+      for (int k = 0; k < np; ++k) npaused += sch.paused[k];
+      return fail(LSQED_ERR_STATE, "batched Lanczos: no replay in flight (" + std::to_string(sch.ndone) + " of " +
+                                       std::to_string(np) + " done, " + std::to_string(npaused) + " paused, " +
+                                       std::to_string(sch.done_r) + " replays processed)");
     }
-    LSQED_HIP(hipEventSynchronize(b->ev[done_r & 1]));
-    const double* ha = b->h_hist + (done_r & 1) * HS;
+    const int r = sch.done_r;
+    LSQED_HIP(hipEventSynchronize(b->ev[r & 1]));
+    const double* ha = b->h_hist + (r & 1) * HS;
     const double* hb = ha + (size_t)kSteps * np;
-    const std::vector<char>& was = act_at[done_r];
     for (int k = 0; k < np; ++k) {
-      if (done[k] || !was[k]) continue;
+      if (!sch.ran(k)) continue;
       auto& al = b->al[k];
       auto& be = b->be[k];
       bool broke = false;
@@ -1187,27 +1239,14 @@ int lsqed_batch_solve(lsqed_batch* b, double* e0, lsqed_stream_t stream) {
       b->resid[k] = res;
       b->steps[k] = m;
       const bool conv = broke || res < b->opts.tol;
-      if (conv || m + kSteps > b->opts.maxit) {
+      const bool stop = conv || m + kSteps > b->opts.maxit;
+      if (stop) {
         b->steps_vec[k] = m;
         b->conv[k] = conv ? 1 : 0;
-        done[k] = 1;
-        ++ndone;
-        if (cur[k]) dirty = true;
-        cur[k] = 0;
-        paused[k] = 0;
-      } else if (paused[k]) {
-        // the prediction failed: resume
-        paused[k] = 0;
-        cur[k] = 1;
-        dirty = true;
-      } else if (prev_res[k] > 0.0 && res < prev_res[k] && res * (res / prev_res[k]) < 0.5 * b->opts.tol) {
-        paused[k] = 1;
-        cur[k] = 0;
-        dirty = true;
       }
-      prev_res[k] = res;
+      sch.result(k, stop, res, b->opts.tol);
     }
-    ++done_r;
+    sch.processed();
   }
   LSQED_HIP(hipStreamSynchronize(b->gs));
   for (int k = 0; k < np; ++k) e0[k] = b->e0[k];

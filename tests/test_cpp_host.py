@@ -33,6 +33,15 @@ def test_cpp_host_cpu(tmp_path):
     assert "12 cases" in out
 
 
+def test_cpp_replay_schedule(tmp_path):
+    """The batched solve's host bookkeeping of graph replays (which replays
+    run which model; pausing a model predicted to converge): scripted and
+    randomised residual sequences, no GPU."""
+    subprocess.run(["make", "-s", "-C", CPP, os.path.join("_build", "test_replay_schedule")], check=True)
+    out = _run("test_replay_schedule", tmp_path, 120)
+    assert "3 cases" in out
+
+
 @pytest.mark.gpu
 def test_cpp_host_gpu(tmp_path):
     """Solver, correlations, m^2 and DSF through the C++ mirror on the GPU,
