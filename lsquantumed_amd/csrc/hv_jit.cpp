@@ -53,6 +53,22 @@ static const char* kJitBody = R"JIT(
 #ifndef PREFETCH
 #define PREFETCH 0
 #endif
+// HB_VEC: a batch's high-bond row bases are read from LDS together before
+// its gathers are issued (the table is padded to a multiple of the batch)
+#ifndef HB_VEC
+#define HB_VEC 0
+#endif
+// BUF_CROSS (with ROWSTAGE): cross gathers are raw buffer loads over the
+// target row; a lane whose bond does not flip passes an offset past the
+// buffer's size, which returns zero without a memory access
+#ifndef BUF_CROSS
+#define BUF_CROSS 0
+#endif
+// BUF_HIGH (with BUF_CROSS and HB_VEC): the high-bond loads too, with the
+// entry's byte offset shared by all of them
+#ifndef BUF_HIGH
+#define BUF_HIGH 0
+#endif
 // ROWSTAGE: blocks of ROW_THREADS threads take whole rows, staging each
 // row's x in LDS, so that the low bonds (targets inside the row) read LDS
 // instead of issuing global gathers
@@ -94,6 +110,29 @@ __device__ __forceinline__ XV xzero() { return 0.0; }
 __device__ __forceinline__ void xmadd(XV& a, double c, XV v) { a += c * v; }
 __device__ __forceinline__ XV xdiag(double d, XV x, XV acc) { return d * x + acc; }
 __device__ __forceinline__ double xdot(XV a, XV b) { return a * b; }
+#endif
+
+#if BUF_CROSS
+typedef int lsq_i32x4 __attribute__((ext_vector_type(4)));
+#if CPLX
+typedef int lsq_ldv __attribute__((ext_vector_type(4)));
+__device__ lsq_ldv lsq_buf_load(lsq_i32x4 rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v4i32");
+#else
+typedef int lsq_ldv __attribute__((ext_vector_type(2)));
+__device__ lsq_ldv lsq_buf_load(lsq_i32x4 rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v2i32");
+#endif
+// buffer descriptor of a row starting at p (wave-uniform): 48-bit base,
+// stride 0, num_records 0x7ffffff0 bytes, data format 32 (gfx9 raw buffer)
+__device__ __forceinline__ lsq_i32x4 row_rsrc(const XV* p) {
+  const unsigned long long a = (unsigned long long)p;
+  lsq_i32x4 r;
+  r.x = __builtin_amdgcn_readfirstlane((int)(unsigned)a);
+  r.y = __builtin_amdgcn_readfirstlane((int)((unsigned)(a >> 32) & 0xffffu));
+  r.z = 0x7ffffff0;
+  r.w = 0x00020000;
+  return r;
+}
+constexpr int kBufNone = (int)0x80000000u;  // offset of a non-flipping lane
 #endif
 
 __device__ __forceinline__ BT rfl(BT v) {
@@ -143,8 +182,9 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   __shared__ unsigned short s_RL1[9 * W1];
   __shared__ BT s_binom[(N + 1) * (NUP + 1)];
   __shared__ double s_f[N];
-  __shared__ BT s_hbase[NBUF][NHIGH + 1];
-  __shared__ double s_hcoef[NBUF][NHIGH + 1];
+  constexpr int HBPAD = ((NHIGH + kG) / kG) * kG;  // > NHIGH, a multiple of the gather batch
+  __shared__ __attribute__((aligned(16))) BT s_hbase[NBUF][HBPAD];
+  __shared__ __attribute__((aligned(16))) double s_hcoef[NBUF][HBPAD];
   __shared__ BT s_cbase[NBUF][NH + 1];
   __shared__ BT s_B;
   __shared__ int s_nact[NBUF], s_cnt0[NBUF], s_cnt1[NBUF];
@@ -374,10 +414,21 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
       // high bonds: row h ^ m, same entry; loads issued in batches of 8
       for (int b0 = 0; b0 < nact; b0 += kG) {
         XV v[kG];
+#if HB_VEC
+        BT hb[kG];
+#pragma unroll
+        for (int u = 0; u < kG; ++u) hb[u] = s_hbase[pb][b0 + u];
+#endif
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
           v[u] = xzero();
+#if HB_VEC && BUF_HIGH
+          if (b0 + u < nact) v[u] = __builtin_bit_cast(XV, lsq_buf_load(row_rsrc(xv + rfl(hb[u])), (int)((unsigned)i * (unsigned)sizeof(XV)), 0, 0));
+#elif HB_VEC
+          if (b0 + u < nact) v[u] = xv[rfl(hb[u]) + (BT)i];
+#else
           if (b0 + u < nact) v[u] = xv[rfl(s_hbase[pb][b0 + u]) + (BT)i];
+#endif
         }
 #pragma unroll
         for (int u = 0; u < kG; ++u)
@@ -443,6 +494,8 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #if MASKED_GATHER
             v[u] = xzero();
             if (on) v[u] = xv[cb[hp] + (BT)i2];
+#elif BUF_CROSS && ROWSTAGE
+            v[u] = __builtin_bit_cast(XV, lsq_buf_load(row_rsrc(xv + cb[hp]), on ? (int)(i2 * (unsigned)sizeof(XV)) : kBufNone, 0, 0));
 #else
             const BT idx = on ? cb[hp] + (BT)i2 : r;
             v[u] = xv[idx];
@@ -586,6 +639,9 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
 #if BATCH
 #define KNAME_NODOT hv_bjit_nodot
 #define KNAME_DOT hv_bjit_dot
+#elif CPLX
+#define KNAME_NODOT hv_jit_c128_nodot
+#define KNAME_DOT hv_jit_c128_dot
 #else
 #define KNAME_NODOT hv_jit_nodot
 #define KNAME_DOT hv_jit_dot
@@ -761,15 +817,16 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<ui
   } else if (cplx) {
     o << "#define WAVES_PER_EU 2\n";
   }
-  // Gathers issued together: 16 for single-model kernels with 32-bit ranks
-  // and at least ~55 global gathers per entry (cross bonds plus about half
-  // the high bonds), else 8.  Results are bit-identical (the sums run in bond
-  // order either way).  H.v at 8 / 16: 4x8 38.9 / 37.2 ms, 8x4 35.2 / 33.5,
-  // 4x7 2.11 / 2.03, 2x12 0.132 / 0.128; 4x6 (49 gathers) 0.129 / 0.130;
-  // chain N=26 0.199 / 0.227; 6x6 (64-bit ranks) 686 / 691; the batched
-  // disorder solve 839 / 810 realisations/s.  LSQED_JIT_GATHER_BATCH
-  // (2..16) overrides.
-  int gather_batch = (!batch && !cplx && N <= 32 && 2 * crq.size() + him.size() >= 110) ? 16 : 8;
+  // Gathers issued together: 16 for real single-model kernels with at least
+  // ~55 global gathers per entry (cross bonds plus about half the high
+  // bonds), else 8.  Results are bit-identical (the sums run in bond order
+  // either way).  H.v at 8 / 16: 4x8 38.9 / 37.2 ms, 8x4 35.2 / 33.5, 4x7
+  // 2.11 / 2.03, 2x12 0.132 / 0.128; 4x6 (49 gathers) 0.129 / 0.130; chain
+  // N=26 0.199 / 0.227; the batched disorder solve 839 / 810 realisations/s.
+  // With the cross gathers as buffer loads: 4x8 37.33 / 36.07 ms (12: 36.68),
+  // 6x6 (64-bit ranks) 656.1 / 638.9 ms (12: 661.3; before the buffer loads
+  // 686 / 691).  LSQED_JIT_GATHER_BATCH (2..16) overrides.
+  int gather_batch = (!batch && !cplx && 2 * crq.size() + him.size() >= 110) ? 16 : 8;
   if (const char* w = getenv("LSQED_JIT_GATHER_BATCH"))
     if (atoi(w) >= 2 && atoi(w) <= 16) gather_batch = atoi(w);
   o << "#define GATHER_BATCH " << gather_batch << "\n";
@@ -779,6 +836,27 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<ui
     if (atoi(w) == 1) o << "#define NT_STORE 1\n";
   if (const char* w = getenv("LSQED_JIT_NT_STAGE"))
     if (atoi(w) == 1) o << "#define NT_STAGE 1\n";
+  // Row-staged kernels: cross gathers as raw buffer loads whose
+  // non-flipping lanes read nothing (BUF_CROSS), a batch's high-bond row bases
+  // read together (HB_VEC) and, for single models, the high-bond loads as
+  // buffer loads too (BUF_HIGH).  H.v with / without the three: 4x8
+  // 36.18 / 36.97 ms, 6x6 660.9 / 677.7 ms, 4x6 0.130 / 0.137 ms, chain N=22
+  // 24.3 / 24.3 us; the batched disorder solve 999 (without BUF_HIGH) / 986
+  // realisations/s (985 with BUF_HIGH).  Results are bit-identical.  Two
+  // alternatives measured slower or equal: non-flipping lanes reading entry 0 of the
+  // target row with ordinary loads (4x8 36.58 ms, 6x6 695 ms), and the row
+  // staged by direct global->LDS loads after the barrier instead of through
+  // registers issued before it (36.91 ms: fewer spilled registers, but the
+  // staging latency is no longer hidden).  LSQED_JIT_BUF_CROSS,
+  // LSQED_JIT_HB_VEC and LSQED_JIT_BUF_HIGH (0/1) override.
+  auto knob = [](const char* name, bool dflt) {
+    const char* w = getenv(name);
+    return w ? atoi(w) == 1 : dflt;
+  };
+  const bool buf_cross = knob("LSQED_JIT_BUF_CROSS", true), hb_vec = knob("LSQED_JIT_HB_VEC", true);
+  if (buf_cross) o << "#define BUF_CROSS 1\n";
+  if (hb_vec) o << "#define HB_VEC 1\n";
+  if (buf_cross && hb_vec && knob("LSQED_JIT_BUF_HIGH", !batch)) o << "#define BUF_HIGH 1\n";
   // The next entry's pattern loaded one entry ahead (row-staged kernels;
   // LSQED_JIT_PREFETCH=0 turns it off): 4x8 H.v 37.13 -> 36.82-36.87 ms, the
   // batched disorder solve 938 -> 957 realisations/s (980 with LOWTAB records,
@@ -856,8 +934,12 @@ static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch
     m->log += "\nhipModuleLoadData failed";
     return m;
   }
-  if (hipModuleGetFunction(&m->fn_dot, m->mod, batch ? "hv_bjit_dot" : "hv_jit_dot") != hipSuccess ||
-      hipModuleGetFunction(&m->fn_nodot, m->mod, batch ? "hv_bjit_nodot" : "hv_jit_nodot") != hipSuccess) {
+  // kernel names: hv_jit_* (one model), hv_jit_c128_* (complex128 entries), hv_bjit_* (batch)
+  const bool cplx = src.find("#define CPLX 1") != std::string::npos;
+  const char* kdot = batch ? "hv_bjit_dot" : (cplx ? "hv_jit_c128_dot" : "hv_jit_dot");
+  const char* knodot = batch ? "hv_bjit_nodot" : (cplx ? "hv_jit_c128_nodot" : "hv_jit_nodot");
+  if (hipModuleGetFunction(&m->fn_dot, m->mod, kdot) != hipSuccess ||
+      hipModuleGetFunction(&m->fn_nodot, m->mod, knodot) != hipSuccess) {
     m->log += "\nhipModuleGetFunction failed";
     return m;
   }
