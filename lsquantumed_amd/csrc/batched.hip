@@ -259,6 +259,34 @@ __global__ __launch_bounds__(kBThreads) void k_batch_update(const BatchProb* __r
 // whose problem offsets and task lengths are all even: four pairs per lane
 // per iteration, their 12 loads issued before any is used; the task's share of
 // ||p||^2 sums the pairs' squares in lane order.
+// NT: nontemporal loads and stores, for batches whose vectors exceed the
+// 256 MB memory-side cache (2^25 doubles); LSQED_BATCH_NT=0/1 overrides.
+// 256-chain disorder workload (842 MB per vector): all-active step 2.559 ->
+// 2.474 ms, solve 1009 -> 1042 realisations/s.
+template <bool NT>
+__device__ __forceinline__ double2 bld2(const double2* p) {
+  if constexpr (NT) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 t = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_double2(t.x, t.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void bst2(double2* p, double2 v) {
+  if constexpr (NT) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    v2 t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
+
+template <bool NT>
 __global__ __launch_bounds__(kBThreads) void k_batch_update2(const BatchProb* __restrict__ probs,
                                                              const BTask* __restrict__ tasks, int ntasks,
                                                              const int* __restrict__ active,
@@ -284,9 +312,9 @@ __global__ __launch_bounds__(kBThreads) void k_batch_update2(const BatchProb* __
       for (int u = 0; u < 4; ++u) {
         const int i = i0 + u * kBThreads + threadIdx.x;
         if (i < end) {
-          vy[u] = y2[i];
-          vq[u] = q2[i];
-          vp[u] = p2[i];
+          vy[u] = bld2<NT>(y2 + i);
+          vq[u] = bld2<NT>(q2 + i);
+          vp[u] = bld2<NT>(p2 + i);
         }
       }
 #pragma unroll
@@ -296,7 +324,7 @@ __global__ __launch_bounds__(kBThreads) void k_batch_update2(const BatchProb* __
           double2 v;
           v.x = a * vy[u].x + b * vq[u].x + c * vp[u].x;
           v.y = a * vy[u].y + b * vq[u].y + c * vp[u].y;
-          p2[i] = v;
+          bst2<NT>(p2 + i, v);
           acc += v.x * v.x;
           acc += v.y * v.y;
         }
@@ -807,7 +835,13 @@ int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, h
   hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_QY), sc(b, S_SQ),
                      sc(b, S_SP), sc(b, S_BETA), hist_alpha(b, j), sc(b, S_COEF));
   const bool aligned = (((uintptr_t)q | (uintptr_t)p | (uintptr_t)b->d_y) & 15u) == 0;
-  hipLaunchKernelGGL(b->pairs && aligned ? k_batch_update2 : k_batch_update, dim3(b->grid), dim3(kBThreads), 0, s,
+  static const int nt_env = [] {
+    const char* e = std::getenv("LSQED_BATCH_NT");
+    return e ? (std::atoi(e) == 1 ? 1 : 0) : -1;
+  }();
+  const bool nt = nt_env >= 0 ? nt_env == 1 : b->total >= ((int64_t)1 << 25);
+  hipLaunchKernelGGL(b->pairs && aligned ? (nt ? k_batch_update2<true> : k_batch_update2<false>) : k_batch_update,
+                     dim3(b->grid), dim3(kBThreads), 0, s,
                      b->d_probs, b->d_tasks, b->ntasks, b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part);
   hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN), 0);
   hipLaunchKernelGGL(k_batch_beta, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_NN), sc(b, S_SQ),

@@ -53,6 +53,32 @@ __device__ __forceinline__ void block_range(int64_t n, int64_t& a, int64_t& b) {
   b = (g + 1) * n / G;
 }
 
+// 16-byte loads and stores of streams that a pass touches once: with NT the
+// nontemporal forms, which do not keep the lines in L2 (chosen for vectors
+// larger than the 256 MB memory-side cache, where nothing would be reused).
+template <bool NT>
+__device__ __forceinline__ double2 ld2(const double2* p) {
+  if constexpr (NT) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 t = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_double2(t.x, t.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void st2(double2* p, double2 v) {
+  if constexpr (NT) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    v2 t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -334,7 +360,7 @@ __global__ __launch_bounds__(kThreads) void k_dcgs_update(const double* __restri
 // rows 2t and 2t+1 with 16-byte loads (columns 16-byte aligned: ld even),
 // halving the load instructions per byte; the arithmetic of each row is
 // that of the one-row kernels.
-template <int MAXM, bool HAS_Y>
+template <int MAXM, bool HAS_Y, bool NT>
 __global__ __launch_bounds__(kThreads) void k_dcgs_proj2(const double* __restrict__ V, int64_t ld, int j,
                                                          const double* __restrict__ u, const double* __restrict__ y,
                                                          int64_t n, double* __restrict__ part) {
@@ -352,9 +378,9 @@ __global__ __launch_bounds__(kThreads) void k_dcgs_proj2(const double* __restric
     double2 vin[MAXM];
 #pragma unroll
     for (int i = 0; i < MAXM; ++i)
-      vin[i] = i < j ? reinterpret_cast<const double2*>(V + (int64_t)i * ld)[r] : make_double2(0.0, 0.0);
-    const double2 ur = U2[r];
-    const double2 yr = HAS_Y ? Y2[r] : make_double2(0.0, 0.0);
+      vin[i] = i < j ? ld2<NT>(reinterpret_cast<const double2*>(V + (int64_t)i * ld) + r) : make_double2(0.0, 0.0);
+    const double2 ur = ld2<NT>(U2 + r);
+    const double2 yr = HAS_Y ? ld2<NT>(Y2 + r) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int i = 0; i < MAXM; ++i) {
       cu[i] += vin[i].x * ur.x;
@@ -454,7 +480,7 @@ __global__ __launch_bounds__(kThreads) void k_dcgs_update2(const double* __restr
   }
 }
 
-template <int MAXM>
+template <int MAXM, bool NT>
 __global__ __launch_bounds__(kThreads) void k_restart2(double* __restrict__ V, int64_t ld, int m, int k,
                                                        const double* __restrict__ S, int64_t n) {
   __shared__ double Ss[MAXM * MAXM];
@@ -465,7 +491,7 @@ __global__ __launch_bounds__(kThreads) void k_restart2(double* __restrict__ V, i
     double2 vin[MAXM];
 #pragma unroll
     for (int j = 0; j < MAXM; ++j)
-      vin[j] = (j < m) ? reinterpret_cast<const double2*>(V + (int64_t)j * ld)[r] : make_double2(0.0, 0.0);
+      vin[j] = (j < m) ? ld2<NT>(reinterpret_cast<const double2*>(V + (int64_t)j * ld) + r) : make_double2(0.0, 0.0);
     for (int c = 0; c < k; ++c) {
       double2 o = make_double2(0.0, 0.0);
 #pragma unroll
@@ -474,7 +500,7 @@ __global__ __launch_bounds__(kThreads) void k_restart2(double* __restrict__ V, i
           o.x += vin[j].x * Ss[c * m + j];
           o.y += vin[j].y * Ss[c * m + j];
         }
-      reinterpret_cast<double2*>(V + (int64_t)c * ld)[r] = o;
+      st2<NT>(reinterpret_cast<double2*>(V + (int64_t)c * ld) + r, o);
     }
   }
 }
@@ -505,7 +531,9 @@ __global__ __launch_bounds__(kThreads) void k_restart(double* __restrict__ V, in
 // On the 4x8 sector (D = 601,080,390; 32 B per row) at 1024 blocks: 3.65 ms
 // (5.27 TB/s) against 3.96 ms for one double per lane over contiguous block
 // ranges; 1 or 2 double2 per lane, 2048-65536 blocks and contiguous ranges
-// with double2 measured 3.59-4.28 ms.
+// with double2 measured 3.59-4.28 ms.  With nontemporal loads and store
+// (NT, large vectors) 3.41 ms; 8 double2 per lane, 512-thread blocks, 2048
+// or 8192 blocks and 32-byte loads were all slower (3.59-4.09 ms).
 constexpr int kUpdU = 4;
 // The same update for vectors that are not 16-byte aligned (an offset into a
 // workspace of odd length): one double per lane over contiguous block ranges.
@@ -525,6 +553,7 @@ __global__ __launch_bounds__(kThreads) void k_lanczos_update_unaligned(const dou
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kThreads) void k_lanczos_update(const double* __restrict__ y,
                                                              const double* __restrict__ q, double* __restrict__ p,
                                                              int64_t n, double a, double b, double c,
@@ -540,9 +569,9 @@ __global__ __launch_bounds__(kThreads) void k_lanczos_update(const double* __res
     for (int u = 0; u < kUpdU; ++u) {
       const int64_t i = t0 + u * kThreads + threadIdx.x;
       if (i < n2) {
-        vy[u] = Y[i];
-        vq[u] = Q[i];
-        vp[u] = P[i];
+        vy[u] = ld2<NT>(Y + i);
+        vq[u] = ld2<NT>(Q + i);
+        vp[u] = ld2<NT>(P + i);
       }
     }
 #pragma unroll
@@ -552,7 +581,7 @@ __global__ __launch_bounds__(kThreads) void k_lanczos_update(const double* __res
         double2 v;
         v.x = a * vy[u].x + b * vq[u].x + c * vp[u].x;
         v.y = a * vy[u].y + b * vq[u].y + c * vp[u].y;
-        P[i] = v;
+        st2<NT>(P + i, v);
         acc += v.x * v.x;
         acc += v.y * v.y;
       }
@@ -895,6 +924,25 @@ static bool two_rows(const Ctx& c, const double* u, const double* y) {
   return on && (c.D % 2 == 0) && (bits & 15) == 0;
 }
 
+// Nontemporal loads and stores for the streaming passes over vectors of
+// 2^25 doubles (256 MB, the memory-side cache) or more, which no later pass
+// finds in a cache anyway; LSQED_NT_STREAMS=0/1 overrides.  4x8 (601,080,390
+// rows) Lanczos update: 3.64 -> 3.41 ms (5.29 -> 5.64 TB/s, standalone sweep:
+// nontemporal loads of y and q alone 3.53 ms, all three loads without the
+// nontemporal store 3.63 ms); in the traced 4x8 Krylov-Schur solve the
+// update 3.65 -> 3.32 ms, the delayed CGS2 projection 11.92 -> 11.36 ms
+// (16 columns) and 15.47 -> 15.21 ms (24), the restart 34.27 -> 33.89 ms;
+// 6x6 update 55.7 -> 54.0 ms.  The delayed CGS2 update pass, which rewrites
+// u and y in place, is left temporal: 17.24 -> 18.15 ms and 19.92 ->
+// 22.28 ms with the nontemporal forms.
+static bool nt_streams(int64_t n) {
+  static const int env = [] {
+    const char* e = std::getenv("LSQED_NT_STREAMS");
+    return e ? (std::atoi(e) != 0 ? 1 : 0) : -1;
+  }();
+  return env >= 0 ? env == 1 : n >= ((int64_t)1 << 25);
+}
+
 // V[:, 0:k] = V[:, 0:m] S[:, 0:k] (S host row-major m x m, first k columns).
 int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
   std::vector<double> Sc((size_t)m * k);
@@ -903,12 +951,16 @@ int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
   LSQED_HIP(hipMemcpyAsync(c.e->dsm, Sc.data(), Sc.size() * sizeof(double), hipMemcpyHostToDevice, c.s));
   const int g = stream_grid(c.D);
   if (two_rows(c, c.e->V, c.e->V)) {
+    const bool nt = nt_streams(c.D);
     if (m <= 16)
-      hipLaunchKernelGGL(k_restart2<16>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+      hipLaunchKernelGGL((nt ? k_restart2<16, true> : k_restart2<16, false>), dim3(g), dim3(kThreads), 0, c.s, c.e->V,
+                         c.D, m, k, c.e->dsm, c.D);
     else if (m <= 32)
-      hipLaunchKernelGGL(k_restart2<32>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+      hipLaunchKernelGGL((nt ? k_restart2<32, true> : k_restart2<32, false>), dim3(g), dim3(kThreads), 0, c.s, c.e->V,
+                         c.D, m, k, c.e->dsm, c.D);
     else
-      hipLaunchKernelGGL(k_restart2<64>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
+      hipLaunchKernelGGL((nt ? k_restart2<64, true> : k_restart2<64, false>), dim3(g), dim3(kThreads), 0, c.s, c.e->V,
+                         c.D, m, k, c.e->dsm, c.D);
   } else if (m <= 16)
     hipLaunchKernelGGL(k_restart<16>, dim3(g), dim3(kThreads), 0, c.s, c.e->V, c.D, m, k, c.e->dsm, c.D);
   else if (m <= 32)
@@ -923,12 +975,13 @@ int restart_transform(Ctx& c, int m, int k, const std::vector<double>& S) {
 template <int MAXM>
 void launch_dcgs_proj(Ctx& c, int G, int j, const double* u, const double* y) {
   if (two_rows(c, u, y)) {
+    const bool nt = nt_streams(c.D);
     if (y)
-      hipLaunchKernelGGL((k_dcgs_proj2<MAXM, true>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D,
-                         c.e->part);
+      hipLaunchKernelGGL((nt ? k_dcgs_proj2<MAXM, true, true> : k_dcgs_proj2<MAXM, true, false>), dim3(G),
+                         dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D, c.e->part);
     else
-      hipLaunchKernelGGL((k_dcgs_proj2<MAXM, false>), dim3(G), dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D,
-                         c.e->part);
+      hipLaunchKernelGGL((nt ? k_dcgs_proj2<MAXM, false, true> : k_dcgs_proj2<MAXM, false, false>), dim3(G),
+                         dim3(kThreads), 0, c.s, c.e->V, c.D, j, u, y, c.D, c.e->part);
     return;
   }
   if (y)
@@ -1333,7 +1386,8 @@ static int lanczos_step_core(lsqed_ham* h, const double* q, double sq, double* p
   const bool aligned = ((reinterpret_cast<uintptr_t>(y) | reinterpret_cast<uintptr_t>(q) |
                          reinterpret_cast<uintptr_t>(p)) & 15) == 0;
   if (aligned)
-    hipLaunchKernelGGL(k_lanczos_update, dim3(R), dim3(kThreads), 0, s, y, q, p, D, sq, -a * sq, -beta * sp, upart);
+    hipLaunchKernelGGL((nt_streams(D) ? k_lanczos_update<true> : k_lanczos_update<false>), dim3(R), dim3(kThreads), 0,
+                       s, y, q, p, D, sq, -a * sq, -beta * sp, upart);
   else
     hipLaunchKernelGGL(k_lanczos_update_unaligned, dim3(R), dim3(kThreads), 0, s, y, q, p, D, sq, -a * sq,
                        -beta * sp, upart);
