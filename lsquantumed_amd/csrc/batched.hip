@@ -554,6 +554,9 @@ struct Group {
   int ntasks = 0;           // row-split tasks per problem
   int part0 = 0;            // first H.v partial of the group
   int gofs = 0;             // offset of its indices in d_gidx
+  // when its problems are one contiguous range: problems [p0, p0 + np) and
+  // their update tasks [t0, t0 + nt) (p0 = -1 otherwise)
+  int p0 = -1, np = 0, t0 = 0, nt = 0;
 };
 
 std::mutex g_tune_mu;
@@ -570,6 +573,7 @@ struct lsqed_batch {
   Shape shape;
   int grid = 0, ntasks = 0;
   bool pairs = false;  // every problem offset and task length even: the update moves double2
+  bool group_chains = false;  // replays run one Lanczos chain per lattice group, on its own stream
   size_t smem = 0;
   int cus = 256;
   // device
@@ -746,6 +750,24 @@ int upload_problems(lsqed_batch* b) {
   LSQED_HIP(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice));
   LSQED_HIP(hipMemcpy(b->d_probs, probs.data(), probs.size() * sizeof(BatchProb), hipMemcpyHostToDevice));
   b->grid = grid_for(b, b->ntasks);
+  // Per-group Lanczos chains need each group's problems (hence its tasks) to
+  // be one contiguous range and a stream per group (LSQED_BATCH_GROUP_CHAINS=0
+  // keeps the lockstep replay).
+  bool contig = spec && b->groups.size() >= 2 && b->groups.size() <= (size_t)lsqed_batch::kSide + 1 && b->side[0];
+  for (Group& g : b->groups) {
+    g.p0 = -1;
+    bool c = !g.idx.empty();
+    for (size_t i = 0; c && i < g.idx.size(); ++i) c = g.idx[i] == g.idx[0] + (int)i;
+    if (c) {
+      g.p0 = g.idx[0];
+      g.np = (int)g.idx.size();
+      g.t0 = probs[g.p0].task0;
+      g.nt = probs[g.p0 + g.np - 1].task1 - g.t0;
+    }
+    contig = contig && c;
+  }
+  const char* gc = std::getenv("LSQED_BATCH_GROUP_CHAINS");
+  b->group_chains = contig && !(gc && std::atoi(gc) == 0);
   const char* pe = std::getenv("LSQED_BATCH_PAIRS");  // 0: the one-row update
   b->pairs = !(pe && std::atoi(pe) == 0);
   for (const BTask& t : tasks) b->pairs = b->pairs && (t.row0 % 2 == 0) && (t.nrows % 2 == 0);
@@ -821,6 +843,16 @@ int launch_hv_batch(lsqed_batch* b, const double* q, double* y, hipStream_t s) {
 
 double* hv_partials(lsqed_batch* b) { return b->shape.kernel == 1 ? b->d_hvpart : b->d_part; }
 
+// Nontemporal update streams for batches whose vectors exceed the 256 MB
+// memory-side cache (LSQED_BATCH_NT=0/1 overrides).
+bool batch_nt(const lsqed_batch* b) {
+  static const int nt_env = [] {
+    const char* e = std::getenv("LSQED_BATCH_NT");
+    return e ? (std::atoi(e) == 1 ? 1 : 0) : -1;
+  }();
+  return nt_env >= 0 ? nt_env == 1 : b->total >= ((int64_t)1 << 25);
+}
+
 // One Lanczos step on the current vector q (scale sq) and previous p; the
 // new vector overwrites p.  Scalars of the step go to history slot j.
 int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, hipStream_t s) {
@@ -835,11 +867,7 @@ int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, h
   hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_QY), sc(b, S_SQ),
                      sc(b, S_SP), sc(b, S_BETA), hist_alpha(b, j), sc(b, S_COEF));
   const bool aligned = (((uintptr_t)q | (uintptr_t)p | (uintptr_t)b->d_y) & 15u) == 0;
-  static const int nt_env = [] {
-    const char* e = std::getenv("LSQED_BATCH_NT");
-    return e ? (std::atoi(e) == 1 ? 1 : 0) : -1;
-  }();
-  const bool nt = nt_env >= 0 ? nt_env == 1 : b->total >= ((int64_t)1 << 25);
+  const bool nt = batch_nt(b);
   hipLaunchKernelGGL(b->pairs && aligned ? (nt ? k_batch_update2<true> : k_batch_update2<false>) : k_batch_update,
                      dim3(b->grid), dim3(kBThreads), 0, s,
                      b->d_probs, b->d_tasks, b->ntasks, b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part);
@@ -882,8 +910,79 @@ void drop_graphs(lsqed_batch* b) {
   }
 }
 
-// kSteps Lanczos steps (q,p roles alternating), the unit of a replay.
+// One Lanczos step of lattice group g alone, on stream st: the same kernels
+// as enqueue_step over the group's problems [p0, p0 + np) and tasks
+// [t0, t0 + nt) (problem indices inside the kernels stay absolute).
+int enqueue_group_step(lsqed_batch* b, const Group& g, double* q, double* p, int j, bool accumulate,
+                       hipStream_t st) {
+  int64_t work = 0;
+  for (const Group& gg : b->groups) work += (int64_t)gg.ntasks * (int64_t)gg.idx.size();
+  BatchJitLaunch L{q,           b->d_y,      b->d_hvpart + g.part0, b->d_poff, b->d_pcoup, b->d_pfields,
+                   b->d_active, b->d_gidx + g.gofs, (int)g.idx.size()};
+  const int rc = launch_hv_batch_jit(g.module, g.h0, L, grid_for(b, work), st);
+  if (rc) return rc;
+  const int p0 = g.p0, np = g.np, sb = (np + 255) / 256;
+  const int grid = grid_for(b, g.nt);
+  if (accumulate)
+    hipLaunchKernelGGL(k_batch_accum, dim3(grid), dim3(kBThreads), 0, st, b->d_probs, b->d_tasks + g.t0, g.nt,
+                       b->d_active, sc(b, S_W + j), sc(b, S_SQ), q, b->d_psi);
+  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, st, b->d_probs + p0, np, b->d_hvpart, sc(b, S_QY) + p0,
+                     1);
+  hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, st, np, b->d_active + p0, sc(b, S_QY) + p0,
+                     sc(b, S_SQ) + p0, sc(b, S_SP) + p0, sc(b, S_BETA) + p0, hist_alpha(b, j) + p0,
+                     sc(b, S_COEF) + 3 * (size_t)p0);
+  const bool aligned = (((uintptr_t)q | (uintptr_t)p | (uintptr_t)b->d_y) & 15u) == 0;
+  if (b->pairs && aligned) {
+    if (batch_nt(b))
+      hipLaunchKernelGGL(k_batch_update2<true>, dim3(grid), dim3(kBThreads), 0, st, b->d_probs, b->d_tasks + g.t0,
+                         g.nt, b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part + g.t0);
+    else
+      hipLaunchKernelGGL(k_batch_update2<false>, dim3(grid), dim3(kBThreads), 0, st, b->d_probs, b->d_tasks + g.t0,
+                         g.nt, b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part + g.t0);
+  } else {
+    hipLaunchKernelGGL(k_batch_update, dim3(grid), dim3(kBThreads), 0, st, b->d_probs, b->d_tasks + g.t0, g.nt,
+                       b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part + g.t0);
+  }
+  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, st, b->d_probs + p0, np, b->d_part, sc(b, S_NN) + p0, 0);
+  hipLaunchKernelGGL(k_batch_beta, dim3(sb), dim3(256), 0, st, np, b->d_active + p0, sc(b, S_NN) + p0,
+                     sc(b, S_SQ) + p0, sc(b, S_SP) + p0, sc(b, S_BETA) + p0, hist_beta(b, j) + p0);
+  return LSQED_OK;
+}
+
+// kSteps Lanczos steps (q,p roles alternating), the unit of a replay.  With
+// group chains every lattice group runs its own kSteps steps on its own
+// stream (the largest on s, the others on the side streams, forked from s
+// and joined back at the end): the groups' problems are independent, so a
+// small group's update and its next steps run while the largest group's H.v
+// still holds most of the GPU, instead of waiting for it at every step.
 int enqueue_replay_steps(lsqed_batch* b, bool accumulate, hipStream_t s) {
+  if (b->group_chains) {
+    const int ng = (int)b->groups.size();
+    int big = 0;
+    for (int gi = 1; gi < ng; ++gi)
+      if (b->groups[gi].nt > b->groups[big].nt) big = gi;
+    std::vector<hipStream_t> st(ng, s);
+    LSQED_HIP(hipEventRecord(b->fork_ev, s));
+    int used = 0;
+    for (int gi = 0; gi < ng; ++gi) {
+      if (gi == big) continue;
+      st[gi] = b->side[used++];
+      LSQED_HIP(hipStreamWaitEvent(st[gi], b->fork_ev, 0));
+    }
+    for (int j = 0; j < kSteps; ++j) {
+      double* q = (j & 1) ? b->d_p : b->d_q;
+      double* p = (j & 1) ? b->d_q : b->d_p;
+      for (int gi = 0; gi < ng; ++gi) {
+        const int rc = enqueue_group_step(b, b->groups[gi], q, p, j, accumulate, st[gi]);
+        if (rc) return rc;
+      }
+    }
+    for (int i = 0; i < used; ++i) {
+      LSQED_HIP(hipEventRecord(b->join_ev[i], b->side[i]));
+      LSQED_HIP(hipStreamWaitEvent(s, b->join_ev[i], 0));
+    }
+    return LSQED_OK;
+  }
   for (int j = 0; j < kSteps; ++j) {
     double* q = (j & 1) ? b->d_p : b->d_q;
     double* p = (j & 1) ? b->d_q : b->d_p;
