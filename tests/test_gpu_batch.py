@@ -141,6 +141,40 @@ def test_batch_schedules_agree(torch):
         np.testing.assert_allclose(v, vals[0], rtol=1e-12, atol=0)
 
 
+def test_batch_group_chains_agree_with_lockstep(O, torch, monkeypatch):
+    """One Lanczos chain per lattice group on its own stream (the default when
+    each group's problems are contiguous) gives bit for bit the energies and
+    step counts of the lockstep replay (LSQED_BATCH_GROUP_CHAINS=0), with and
+    without ground-state vectors; a batch whose groups interleave runs the
+    lockstep replay and still meets the oracle."""
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    hams = disorder_chains([12, 14, 16], [3, 2, 3], J1=1.0, J2=0.2, W=2.0, seed=11)
+    runs = {}
+    for mode in ("chains", "lockstep"):
+        if mode == "lockstep":
+            monkeypatch.setenv("LSQED_BATCH_GROUP_CHAINS", "0")
+        for kv in (False, True):
+            say("group chains", mode, "keep_vectors", kv)
+            b = BatchSolver(hams, tol=1e-12, hv_kernel=BatchSolver.HV_SPECIALISED, keep_vectors=kv)
+            e0 = b.solve()
+            runs[(mode, kv)] = (e0.copy(), [b.info(k)[0] for k in range(len(hams))])
+            b.destroy()
+    monkeypatch.delenv("LSQED_BATCH_GROUP_CHAINS", raising=False)
+    for kv in (False, True):
+        assert np.array_equal(runs[("chains", kv)][0], runs[("lockstep", kv)][0]), kv
+        assert runs[("chains", kv)][1] == runs[("lockstep", kv)][1], kv
+    for k, h in enumerate(hams):
+        ref = oracle_ground(O, h.basis.nspins, 1.0, 0.2, h.hi)
+        assert abs(runs[("chains", False)][0][k] - ref) <= 1e-11 * abs(ref), (k, ref)
+    mixed = [hams[0], hams[3], hams[1], hams[5], hams[4]]  # N = 12, 14, 12, 16, 14: groups interleave
+    b = BatchSolver(mixed, tol=1e-12, hv_kernel=BatchSolver.HV_SPECIALISED)
+    e0 = b.solve()
+    b.destroy()
+    for k, h in enumerate(mixed):
+        ref = oracle_ground(O, h.basis.nspins, 1.0, 0.2, h.hi)
+        assert abs(e0[k] - ref) <= 1e-11 * abs(ref), (k, ref)
+
+
 def test_batch_resolve_after_new_fields(O, torch):
     """Fields set on the models between solves take effect at the next solve
     (the reference rebuilds the disorder per realisation, main_disorder.cpp:83-84)."""
