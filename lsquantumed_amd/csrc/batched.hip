@@ -43,7 +43,16 @@
 namespace lsqed {
 
 constexpr int kBThreads = 256;
-constexpr int kSteps = 8;          // Lanczos steps per graph replay (even)
+// Lanczos steps per graph replay (even).  Shorter replays let a realisation
+// stop nearer the step where it converges; longer ones cost fewer host
+// round trips.  256-chain disorder workload, steps per replay 2 / 4 / 6 / 8 /
+// 12: 1084-1090 / 1093-1095 / 1086 / 1074-1077 / 1058 realisations/s
+// (mean Lanczos steps 82.1 / 83.0 / 84.2 / 85.0 / 87.0).
+#ifndef LSQED_BATCH_STEPS
+#define LSQED_BATCH_STEPS 4
+#endif
+constexpr int kSteps = LSQED_BATCH_STEPS;
+static_assert(kSteps >= 2 && kSteps % 2 == 0, "a replay must end where it started (q and p roles)");
 constexpr int kObsStride = LSQED_BATCH_OBS_STRIDE;
 
 // One problem of the batch (device-resident descriptor).
