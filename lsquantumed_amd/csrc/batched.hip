@@ -205,19 +205,43 @@ __global__ void k_batch_reduce(const BatchProb* __restrict__ probs, int nprob, c
   if (threadIdx.x == 0) out[k] = tot;
 }
 
-// alpha_k = sq^2 (q.y); coefficients of p <- sq*y - alpha*sq*q - beta*sp*p.
-__global__ void k_batch_alpha(int nprob, const int* __restrict__ active, const double* __restrict__ qy,
-                              const double* __restrict__ sq, const double* __restrict__ sp,
-                              const double* __restrict__ beta, double* __restrict__ alpha_out,
-                              double* __restrict__ coef) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// The per-problem sum of k_batch_reduce fused with the scalar step that
+// consumes it (one launch where there were two).  After the H.v (AB = 0):
+// alpha_k = sq^2 (q.y) and the coefficients of p <- sq*y - alpha*sq*q -
+// beta*sp*p; after the update (AB = 1): beta' = ||p|| and the scales of the
+// next step, sp <- sq, sq <- 1/beta'.
+template <int AB>
+__global__ void k_batch_reduce_ab(const BatchProb* __restrict__ probs, int nprob, const double* __restrict__ part,
+                                  double* __restrict__ out, const int* __restrict__ active, double* __restrict__ sq,
+                                  double* __restrict__ sp, double* __restrict__ beta, double* __restrict__ hist,
+                                  double* __restrict__ coef) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
   if (k >= nprob) return;
-  const double s = sq[k];
-  const double a = active[k] ? s * s * qy[k] : 0.0;
-  alpha_out[k] = a;
-  coef[3 * k + 0] = s;
-  coef[3 * k + 1] = -a * s;
-  coef[3 * k + 2] = -beta[k] * sp[k];
+  const int t0 = AB == 0 ? probs[k].hv0 : probs[k].task0, t1 = AB == 0 ? probs[k].hv1 : probs[k].task1;
+  double acc = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) acc += part[t];
+  const double tot = block_sum(acc, red);
+  if (threadIdx.x != 0) return;
+  out[k] = tot;
+  if (AB == 0) {
+    const double s = sq[k];
+    const double a = active[k] ? s * s * tot : 0.0;
+    hist[k] = a;
+    coef[3 * k + 0] = s;
+    coef[3 * k + 1] = -a * s;
+    coef[3 * k + 2] = -beta[k] * sp[k];
+  } else {
+    if (!active[k]) {
+      hist[k] = 0.0;
+      return;
+    }
+    const double b = sqrt(tot);
+    hist[k] = b;
+    sp[k] = sq[k];
+    sq[k] = b > 0.0 ? 1.0 / b : 0.0;
+    beta[k] = b;
+  }
 }
 
 // p <- a_k y + b_k q + c_k p ; normpart[t] = the task's share of ||p||^2.
@@ -342,23 +366,6 @@ __global__ __launch_bounds__(kBThreads) void k_batch_update2(const BatchProb* __
     const double tot = block_sum(acc, red);
     if (threadIdx.x == 0) normpart[t] = tot;
   }
-}
-
-// beta' = ||p||; scales for the next step: sp <- sq, sq <- 1/beta'.
-__global__ void k_batch_beta(int nprob, const int* __restrict__ active, const double* __restrict__ nn,
-                             double* __restrict__ sq, double* __restrict__ sp, double* __restrict__ beta,
-                             double* __restrict__ beta_out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nprob) return;
-  if (!active[k]) {
-    beta_out[k] = 0.0;
-    return;
-  }
-  const double b = sqrt(nn[k]);
-  beta_out[k] = b;
-  sp[k] = sq[k];
-  sq[k] = b > 0.0 ? 1.0 / b : 0.0;
-  beta[k] = b;
 }
 
 // psi_k += w_k * sq_k * q_k (second pass: the Ritz vector, one Lanczos
@@ -866,23 +873,20 @@ bool batch_nt(const lsqed_batch* b) {
 // new vector overwrites p.  Scalars of the step go to history slot j.
 int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, hipStream_t s) {
   const int np = b->nprob;
-  const int sb = (np + 255) / 256;
   const int rc = launch_hv_batch(b, q, b->d_y, s);
   if (rc) return rc;
   if (accumulate)
     hipLaunchKernelGGL(k_batch_accum, dim3(b->grid), dim3(kBThreads), 0, s, b->d_probs, b->d_tasks, b->ntasks,
                        b->d_active, sc(b, S_W + j), sc(b, S_SQ), q, b->d_psi);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, hv_partials(b), sc(b, S_QY), 1);
-  hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_QY), sc(b, S_SQ),
-                     sc(b, S_SP), sc(b, S_BETA), hist_alpha(b, j), sc(b, S_COEF));
+  hipLaunchKernelGGL(k_batch_reduce_ab<0>, dim3(np), dim3(256), 0, s, b->d_probs, np, hv_partials(b), sc(b, S_QY),
+                     b->d_active, sc(b, S_SQ), sc(b, S_SP), sc(b, S_BETA), hist_alpha(b, j), sc(b, S_COEF));
   const bool aligned = (((uintptr_t)q | (uintptr_t)p | (uintptr_t)b->d_y) & 15u) == 0;
   const bool nt = batch_nt(b);
   hipLaunchKernelGGL(b->pairs && aligned ? (nt ? k_batch_update2<true> : k_batch_update2<false>) : k_batch_update,
                      dim3(b->grid), dim3(kBThreads), 0, s,
                      b->d_probs, b->d_tasks, b->ntasks, b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN), 0);
-  hipLaunchKernelGGL(k_batch_beta, dim3(sb), dim3(256), 0, s, np, b->d_active, sc(b, S_NN), sc(b, S_SQ),
-                     sc(b, S_SP), sc(b, S_BETA), hist_beta(b, j));
+  hipLaunchKernelGGL(k_batch_reduce_ab<1>, dim3(np), dim3(256), 0, s, b->d_probs, np, b->d_part, sc(b, S_NN),
+                     b->d_active, sc(b, S_SQ), sc(b, S_SP), sc(b, S_BETA), hist_beta(b, j), nullptr);
   return LSQED_OK;
 }
 
@@ -930,16 +934,14 @@ int enqueue_group_step(lsqed_batch* b, const Group& g, double* q, double* p, int
                    b->d_active, b->d_gidx + g.gofs, (int)g.idx.size()};
   const int rc = launch_hv_batch_jit(g.module, g.h0, L, grid_for(b, work), st);
   if (rc) return rc;
-  const int p0 = g.p0, np = g.np, sb = (np + 255) / 256;
+  const int p0 = g.p0, np = g.np;
   const int grid = grid_for(b, g.nt);
   if (accumulate)
     hipLaunchKernelGGL(k_batch_accum, dim3(grid), dim3(kBThreads), 0, st, b->d_probs, b->d_tasks + g.t0, g.nt,
                        b->d_active, sc(b, S_W + j), sc(b, S_SQ), q, b->d_psi);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, st, b->d_probs + p0, np, b->d_hvpart, sc(b, S_QY) + p0,
-                     1);
-  hipLaunchKernelGGL(k_batch_alpha, dim3(sb), dim3(256), 0, st, np, b->d_active + p0, sc(b, S_QY) + p0,
-                     sc(b, S_SQ) + p0, sc(b, S_SP) + p0, sc(b, S_BETA) + p0, hist_alpha(b, j) + p0,
-                     sc(b, S_COEF) + 3 * (size_t)p0);
+  hipLaunchKernelGGL(k_batch_reduce_ab<0>, dim3(np), dim3(256), 0, st, b->d_probs + p0, np, b->d_hvpart,
+                     sc(b, S_QY) + p0, b->d_active + p0, sc(b, S_SQ) + p0, sc(b, S_SP) + p0, sc(b, S_BETA) + p0,
+                     hist_alpha(b, j) + p0, sc(b, S_COEF) + 3 * (size_t)p0);
   const bool aligned = (((uintptr_t)q | (uintptr_t)p | (uintptr_t)b->d_y) & 15u) == 0;
   if (b->pairs && aligned) {
     if (batch_nt(b))
@@ -952,9 +954,9 @@ int enqueue_group_step(lsqed_batch* b, const Group& g, double* q, double* p, int
     hipLaunchKernelGGL(k_batch_update, dim3(grid), dim3(kBThreads), 0, st, b->d_probs, b->d_tasks + g.t0, g.nt,
                        b->d_active, sc(b, S_COEF), b->d_y, q, p, b->d_part + g.t0);
   }
-  hipLaunchKernelGGL(k_batch_reduce, dim3(np), dim3(256), 0, st, b->d_probs + p0, np, b->d_part, sc(b, S_NN) + p0, 0);
-  hipLaunchKernelGGL(k_batch_beta, dim3(sb), dim3(256), 0, st, np, b->d_active + p0, sc(b, S_NN) + p0,
-                     sc(b, S_SQ) + p0, sc(b, S_SP) + p0, sc(b, S_BETA) + p0, hist_beta(b, j) + p0);
+  hipLaunchKernelGGL(k_batch_reduce_ab<1>, dim3(np), dim3(256), 0, st, b->d_probs + p0, np, b->d_part,
+                     sc(b, S_NN) + p0, b->d_active + p0, sc(b, S_SQ) + p0, sc(b, S_SP) + p0, sc(b, S_BETA) + p0,
+                     hist_beta(b, j) + p0, nullptr);
   return LSQED_OK;
 }
 
