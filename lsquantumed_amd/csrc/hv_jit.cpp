@@ -509,7 +509,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-#if LOWTAB
+#if LOWTAB == 1
       // low bonds: inside row h, partner entries from the record (a parallel
       // pair reads the zero entry, so every flipping bond is one LDS read and
       // one FMA, summed in bond order as below)
@@ -518,6 +518,28 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
         const unsigned int w = rw[b >> 1];
         const unsigned int j = (b & 1) ? (w >> 16) : (w & 0xffffu);
         xmadd(acc, jh_[LOWFK[b]], s_x[j]);
+      }
+      cc0 += (int)((rw[LWORD] >> 16) & 0xffu);
+      cc1 += (int)(rw[LWORD] >> 24);
+#elif LOWTAB == 2
+      // low bonds in bond order: those with a site in each byte of the
+      // pattern (two rank lookups) from the record, the others by one lookup;
+      // a parallel pair reads the zero entry either way, and the antiparallel
+      // counts come from the record
+#pragma unroll
+      for (int b = 0; b < NLOW; ++b) {
+        if (!flip[LOWK[b]]) continue;
+        if (LOWREC[b] >= 0) {
+          const unsigned int w = rw[LOWREC[b] >> 1];
+          const unsigned int j = (LOWREC[b] & 1) ? (w >> 16) : (w & 0xffffu);
+          xmadd(acc, jh_[LOWK[b]], s_x[j]);
+        } else {
+          const unsigned int m = LOWM[b];
+          const unsigned int l2 = l ^ m;
+          const unsigned int i2 =
+              LOWT[b] == 0 ? s_RL0[l2 & 255u] + t1 : t0 + (unsigned int)s_RL1[k0 * W1 + (l2 >> 8)];
+          xmadd(acc, jh_[LOWK[b]], s_x[__builtin_popcount(l & m) == 1 ? i2 : (unsigned int)RMAX]);
+        }
       }
       cc0 += (int)((rw[LWORD] >> 16) & 0xffu);
       cc1 += (int)(rw[LWORD] >> 24);
@@ -734,26 +756,32 @@ static std::string dbl(double v) {
 // Low-bond records (LOWTAB) for row-staged kernels with L <= 14: on for the
 // batched kernel, whose chains of N = 18-22 spend most VALU issue on the
 // per-bond rank arithmetic of their 25 low bonds (VALU 66% busy on the N = 22
-// launch); LSQED_JIT_LOWTAB=0/1 overrides for both kernels.  Off for single
-// models: the 4x8 H.v took 38.92 ms with records against 37.2 ms without, 6x6
-// 726 against 686 ms (the extra record loads on an HBM-bound kernel).
-static bool jit_lowtab(bool batch, int L) {
-  if (L > 14) return false;
+// launch).  Mode 1 records every flipping low bond (64 B per pattern for an
+// N = 22 chain), mode 2 -- the batched default -- only those with a site in
+// each byte of the pattern (two rank lookups each; 3 of the 25, 16 B per
+// pattern), the others costing one lookup: the 256-chain disorder solve 1099
+// (mode 1) / 1127 (mode 2) realisations/s, the record loads having moved the
+// bound to the texture addresser (74% busy).  Off for single models: the 4x8
+// H.v took 38.92 ms with mode-1 records against 37.2 ms without, 6x6 726
+// against 686 ms, and 46.5 ms with mode 2 against 36.1 (extra loads on an
+// HBM-bound kernel).  LSQED_JIT_LOWTAB=0/1/2 overrides for both kernels.
+static int jit_lowtab(bool batch, int L) {
+  if (L > 14) return 0;
   const char* e = getenv("LSQED_JIT_LOWTAB");
-  return e ? atoi(e) == 1 : batch;
+  const int v = e ? atoi(e) : (batch ? 2 : 0);
+  return (v == 1 || v == 2) ? v : 0;
 }
 
 // Records of the LOWTAB kernel, one per low pattern l in U's order (popcount
 // class, then numeric = colex order), `recq` x 4 words each: words
-// 0..(nf+1)/2-1 hold the row indices of l ^ m for the flipping low bonds m
-// (two 16-bit halves per word, bond order), or `zslot` when the bond's sites
-// are parallel; word (nf+1)/2 holds l | cc0 << 16 | cc1 << 24, the counts of
-// antiparallel low bonds of kind 0 and 1 (every low bond, flipping or not).
+// 0..(nf+1)/2-1 hold the row indices of l ^ m for the low bonds m listed in
+// `f` (flipping bonds: all of them, or with LOWTAB = 2 those with a site in
+// each byte of l; two 16-bit halves per word, bond order), or `zslot` when
+// the bond's sites are parallel; word (nf+1)/2 holds l | cc0 << 16 | cc1 << 24,
+// the counts of antiparallel low bonds of kind 0 and 1 (every low bond,
+// flipping or not).
 static std::vector<uint32_t> lowtab_records(int L, const std::vector<unsigned>& lowm, const std::vector<int>& lowk,
-                                            const int flip[2], int recq, unsigned zslot) {
-  std::vector<int> f;
-  for (size_t b = 0; b < lowm.size(); ++b)
-    if (flip[lowk[b]]) f.push_back((int)b);
+                                            const std::vector<int>& f, int recq, unsigned zslot) {
   const int lword = ((int)f.size() + 1) / 2;
   auto rank = [&](unsigned v) {
     uint32_t r = 0;
@@ -870,15 +898,22 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<ui
     const uint64_t rmax = binom_u64(L, L / 2);
     o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L) << "\nconstexpr int RMAX = " << rmax
       << ";\n";
-    if (jit_lowtab(batch, L)) {
-      std::vector<int> lowfk;
+    if (const int mode = jit_lowtab(batch, L)) {
+      // mode 1: every flipping low bond in the record; mode 2: only those
+      // with a site in each byte of the pattern (lowt 2), the others computed
+      std::vector<int> f, lowfk, lowrec(lowm.size(), -1);
       for (size_t b = 0; b < lowm.size(); ++b)
-        if (rp.flip[lowk[b]]) lowfk.push_back(lowk[b]);
-      const int nf = (int)lowfk.size(), lword = (nf + 1) / 2, recq = (lword + 1 + 3) / 4;
-      o << "#define LOWTAB 1\nconstexpr int NLOWF = " << nf << ", LWORD = " << lword << ", RECQ = " << recq
-        << ";\n";
+        if (rp.flip[lowk[b]] && (mode == 1 || lowt[b] == 2)) {
+          lowrec[b] = (int)f.size();
+          f.push_back((int)b);
+          lowfk.push_back(lowk[b]);
+        }
+      const int nf = (int)f.size(), lword = (nf + 1) / 2, recq = (lword + 1 + 3) / 4;
+      o << "#define LOWTAB " << mode << "\nconstexpr int NLOWF = " << nf << ", LWORD = " << lword
+        << ", RECQ = " << recq << ";\n";
       emit_array(o, "LOWFK", lowfk, "int");
-      if (lowtab) *lowtab = lowtab_records(L, lowm, lowk, rp.flip, recq, (unsigned)rmax);
+      emit_array(o, "LOWREC", lowrec, "int");
+      if (lowtab) *lowtab = lowtab_records(L, lowm, lowk, f, recq, (unsigned)rmax);
     }
   }
   o << "constexpr int N = " << N << ", NUP = " << nup << ", L = " << L << ", NH = " << NH
@@ -969,7 +1004,10 @@ static std::shared_ptr<JitModule> jit_module(const lsqed_ham* h, bool batch, boo
   if (it != g_jit_cache.end()) return it->second;
   std::shared_ptr<JitModule> m = jit_compile(src, batch);
   std::vector<uint32_t> tab;
-  if (m->ok && src.find("#define LOWTAB 1") != std::string::npos) {
+  // (the kernel body's own fallback reads "#define LOWTAB 0")
+  const bool lowtab = src.find("#define LOWTAB 1\n") != std::string::npos ||
+                      src.find("#define LOWTAB 2\n") != std::string::npos;
+  if (m->ok && lowtab) {
     (void)jit_source(h, batch, cplx, &tab);
     const size_t bytes = tab.size() * sizeof(uint32_t);
     if (tab.empty() || hipMalloc(&m->d_lowtab, bytes) != hipSuccess ||

@@ -65,7 +65,7 @@ static int generate(int lat, int N, int lx, int ly) {
 // Low-bond records of a LOWTAB source against brute force: patterns of each
 // popcount class listed in numeric order, a partner's index looked up in that
 // list (no rank formula), the antiparallel counts recounted bond by bond.
-static int check_records(int lat, int N, int lx, int ly, bool batch, double J2) {
+static int check_records(int lat, int N, int lx, int ly, bool batch, double J2, int mode) {
   lsqed_ham h;
   h.N = N;
   h.nup = N / 2;
@@ -76,7 +76,7 @@ static int check_records(int lat, int N, int lx, int ly, bool batch, double J2) 
   EXPECT(bonds(lat, N, lx, ly, h));
   std::vector<uint32_t> rec;
   const std::string s = lsqed::jit_source(&h, batch, false, &rec);
-  EXPECT(s.find("#define LOWTAB 1") != std::string::npos);
+  EXPECT(s.find(mode == 1 ? "#define LOWTAB 1\n" : "#define LOWTAB 2\n") != std::string::npos);
   const int L = lsqed::split_low_bits(&h, batch);
   std::vector<unsigned> m;
   std::vector<int> k, flipping;
@@ -86,7 +86,9 @@ static int check_records(int lat, int N, int lx, int ly, bool batch, double J2) 
       if (a == c || c >= L) continue;
       m.push_back((1u << a) | (1u << c));
       k.push_back(kind);
-      if (h.J[kind] != 0.0) flipping.push_back((int)m.size() - 1);
+      // mode 1 keeps every flipping bond in the record, mode 2 only those with
+      // one site in each byte of the pattern
+      if (h.J[kind] != 0.0 && (mode == 1 || (a < 8 && c >= 8))) flipping.push_back((int)m.size() - 1);
     }
   const int nf = (int)flipping.size(), lword = (nf + 1) / 2, recq = (lword + 4) / 4;
   std::vector<std::vector<unsigned>> cls(L + 1);
@@ -121,13 +123,16 @@ static int check_records(int lat, int N, int lx, int ly, bool batch, double J2) 
 }
 
 int main() {
-  int nrec = check_records(LSQED_CHAIN1D, 22, 0, 0, true, 0.2);
-  nrec += check_records(LSQED_CHAIN1D, 18, 0, 0, true, 0.0);  // NNN bonds do not flip, still counted
+  int nrec = check_records(LSQED_CHAIN1D, 22, 0, 0, true, 0.2, 2);  // the batched kernel's default
+  nrec += check_records(LSQED_CHAIN1D, 18, 0, 0, true, 0.0, 2);      // NNN bonds do not flip, still counted
   setenv("LSQED_JIT_LOWTAB", "1", 1);
-  nrec += check_records(LSQED_SQUARE2D, 32, 4, 8, false, 0.5);
-  nrec += check_records(LSQED_HONEYCOMB2D, 24, 4, 6, false, 0.4);
+  nrec += check_records(LSQED_CHAIN1D, 22, 0, 0, true, 0.2, 1);
+  nrec += check_records(LSQED_SQUARE2D, 32, 4, 8, false, 0.5, 1);
+  nrec += check_records(LSQED_HONEYCOMB2D, 24, 4, 6, false, 0.4, 1);
+  setenv("LSQED_JIT_LOWTAB", "2", 1);
+  nrec += check_records(LSQED_SQUARE2D, 32, 4, 8, false, 0.5, 2);
   unsetenv("LSQED_JIT_LOWTAB");
-  EXPECT(nrec == 4);
+  EXPECT(nrec == 6);
   int nsrc = 0;
   for (int N = 6; N <= 40; N += 2) nsrc += generate(LSQED_CHAIN1D, N, 0, 0);
   const int sq[][2] = {{2, 4}, {3, 4}, {4, 4}, {4, 5}, {4, 6}, {2, 12}, {4, 7}, {4, 8}, {8, 4}, {6, 6}, {4, 10}, {6, 1}};

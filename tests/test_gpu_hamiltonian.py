@@ -246,6 +246,25 @@ def test_hv_graph_capture_replays(O, torch, kernel):
     assert torch.equal(y, ref2), kernel
 
 
+@pytest.mark.parametrize("lat,N,mag,lx,ly,builds", [
+    ("chain", 16, 0, 0, 0, ("build_diag", "build_off_diag")),
+    ("chain", 22, 1, 0, 0, ("build_diag", "build_off_diag", "build_disorder")),
+    ("square", 16, 0, 4, 4, ("build_diag", "build_off_diag")),
+    ("square", 24, 0, 4, 6, ("build_diag", "build_off_diag")),
+    ("honeycomb", 24, 0, 4, 6, ("build_diag", "build_off_diag", "build_disorder")),
+    ("chain", 40, -18, 0, 0, ("build_diag", "build_off_diag", "build_disorder")),
+    ("square", 36, 17, 6, 6, ("build_diag", "build_off_diag"))])
+def test_hv_auto_kernel_is_specialised(torch, lat, N, mag, lx, ly, builds):
+    """The automatic choice resolves to the hipRTC-specialised row split on
+    every lattice kind, with and without fields, 32- and 64-bit ranks and
+    row splits of L = 10..16: a kernel that fails to compile would otherwise
+    fall back to the unspecialised kernels unnoticed."""
+    hi = np.linspace(-0.5, 0.5, N) if "build_disorder" in builds else None
+    h = make(lat, N, mag, 1.0, 1.0, 0.5, 0.5, lx, ly, builds=builds, hi=hi)
+    say("auto kernel", lat, N, mag, h.hv_kernel())
+    assert h.hv_kernel() == 3
+
+
 @pytest.mark.parametrize("lat,N,mag,lx,ly", [("chain", 38, -17, 0, 0), ("chain", 40, -18, 0, 0),
                                               ("chain", 40, 18, 0, 0), ("square", 40, -17, 4, 10)])
 @pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
