@@ -175,6 +175,28 @@ def test_batch_group_chains_agree_with_lockstep(O, torch, monkeypatch):
         assert abs(e0[k] - ref) <= 1e-11 * abs(ref), (k, ref)
 
 
+def test_batch_low_bond_record_modes_agree(O, torch, monkeypatch):
+    """The batched H.v's low bonds by rank arithmetic (LSQED_JIT_LOWTAB=0),
+    from full per-pattern records (1) or from records of the bonds needing
+    two rank lookups only (2, the default): every bond's term is summed in
+    bond order either way, so the energies and step counts agree bit for bit."""
+    from lsquantumed_amd.batch import BatchSolver, disorder_chains
+    hams = disorder_chains([14, 18, 20], [2, 2, 2], J1=1.0, J2=0.2, W=2.0, seed=5)
+    runs = []
+    for mode in ("0", "1", "2"):
+        monkeypatch.setenv("LSQED_JIT_LOWTAB", mode)
+        say("low-bond records mode", mode)
+        b = BatchSolver(hams, tol=1e-12, hv_kernel=BatchSolver.HV_SPECIALISED)
+        runs.append((b.solve().copy(), [b.info(k)[0] for k in range(len(hams))]))
+        b.destroy()
+    monkeypatch.delenv("LSQED_JIT_LOWTAB", raising=False)
+    for e0, steps in runs[1:]:
+        assert np.array_equal(e0, runs[0][0]) and steps == runs[0][1]
+    for k, h in enumerate(hams):
+        ref = oracle_ground(O, h.basis.nspins, 1.0, 0.2, h.hi)
+        assert abs(runs[0][0][k] - ref) <= 1e-11 * abs(ref), (k, ref)
+
+
 def test_batch_resolve_after_new_fields(O, torch):
     """Fields set on the models between solves take effect at the next solve
     (the reference rebuilds the disorder per realisation, main_disorder.cpp:83-84)."""
