@@ -1035,18 +1035,19 @@ static int jit_grid_of(lsqed_ham* h, const JitModule* m) {
   SplitTables t;
   if (split_tables(h, t) != LSQED_OK) return 0;
   if (m->rowstage) {
-    // two rows per block when there are at least 2^15 rows, else one
-    // (LSQED_JIT_ROWS_PER_BLOCK overrides): a block's once-per-block work --
-    // its tables, and the stores of the 12 VGPRs the kernel spills at four
-    // waves per SIMD (3.2 GB of scratch writes per 4x8 product at one row
-    // per block) -- is shared by two rows, while the dispatcher still
-    // balances the uneven row lengths.  Rows per block on 4x8 / 6x6: 1: 39.5
-    // / 701.7 ms, 2: 39.1 / 688.8 ms, 4: 38.6 / 714.1 ms, 8: 40.8 ms on 4x8;
-    // on 4x6 (2^10 rows) 2 was 43% slower than 1.  A persistent grid (one
-    // block per CU dealing rows cyclically) measured 7% slower: a fixed share
-    // of rows per CU is unevenly long.
+    // one row per block (LSQED_JIT_ROWS_PER_BLOCK overrides).  With each
+    // XCD's rows dealt longest first the dispatcher itself then schedules
+    // longest-first, and the launch ends on the shortest rows: 4x8 34.73-
+    // 34.84 ms against 35.67-35.72 with two rows per block, 6x6 625.1 against
+    // 636.2 ms, 8x4 29.33 against 30.01 ms.  In h order two rows per block had
+    // been better (a block's once-per-block work -- its tables and the stores
+    // of the 12 VGPRs the kernel spills at four waves per SIMD -- shared by
+    // two rows): rows per block on 4x8 / 6x6 1: 39.5 / 701.7 ms, 2: 39.1 /
+    // 688.8 ms, 4: 38.6 / 714.1 ms, 8: 40.8 ms on 4x8 at the time.  A
+    // persistent grid (one block per CU dealing rows cyclically) measured 7%
+    // slower: a fixed share of rows per CU is unevenly long.
     const char* e = getenv("LSQED_JIT_ROWS_PER_BLOCK");
-    const int k = e ? std::max(1, std::min(64, atoi(e))) : (t.nrows_ord >= 32768 ? 2 : 1);
+    const int k = e ? std::max(1, std::min(64, atoi(e))) : 1;
     const int per_xcd = t.nrows_ord / 8;
     return 8 * std::max(1, (per_xcd + k - 1) / k);
   }
