@@ -320,8 +320,10 @@ void split_plan_destroy(lsqed_ham* h) {
 // (FETCH_SIZE equal within 0.3%).  With 2048-entry items the 4x8 J1-J2 H.v
 // went from 44.9 to 43.6 ms at ko = 5 (150 MB groups), the 6x6 one from 781.6
 // to 780.2 ms; with whole rows staged in LDS (the default specialised kernel)
-// h order is best (4x8: 40.96 ms at ko = 0, 41.78 at ko = 5), so grouping is
-// off unless LSQED_SPLIT_OUTER (ko) or LSQED_SPLIT_GROUP_MB asks for it.
+// h order is best (4x8: 40.96 ms at ko = 0, 41.78 at ko = 5; with each XCD's
+// rows dealt longest first and one row per block, 34.75-34.80 ms at ko = 0
+// against 35.48 at ko = 3), so grouping is off unless LSQED_SPLIT_OUTER (ko)
+// or LSQED_SPLIT_GROUP_MB asks for it.
 static int outer_site_count(const lsqed_ham* h, int NH) {
   if (const char* e = std::getenv("LSQED_SPLIT_OUTER")) return std::max(0, std::min(NH, std::atoi(e)));
   const char* g = std::getenv("LSQED_SPLIT_GROUP_MB");
@@ -373,6 +375,10 @@ static void order_items(const lsqed_ham* h, int L, int NH, int ko, std::vector<u
   // stay inside the sub-cube they span (high bonds with both ends in it,
   // cross bonds at one of its sites): consecutive rows, which one XCD holds
   // at the same time, then read more of their targets from each other.
+  // Off by default: on 4x8 J1-J2 with rows dealt longest first, k = 4 / 7 /
+  // 10 measured 34.77 / 34.61 / 34.75 ms against 34.75-34.80 without, within
+  // the run-to-run spread (the key changes which rows share an XCD and their
+  // order among rows of equal length).
   if (const char* e = std::getenv("LSQED_SPLIT_FAST")) {
     const int k = std::max(0, std::min((int)inner.size(), std::atoi(e)));
     std::vector<int> fast;
