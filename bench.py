@@ -727,7 +727,7 @@ def main():
     ap.add_argument("--maxit", type=int, default=600,
                     help="with --lanczos-6x6: at most this many Lanczos steps (a few for a profile)")
     ap.add_argument("--krylov-4x8", action="store_true",
-                    help="only Krylov-Schur nev=4 ncv=20 on the 4x8 sector (101 GB basis; about a minute)")
+                    help="only Krylov-Schur nev=4 ncv=20 on the 4x8 sector (101 GB basis + a 4.8 GB spare; about a minute)")
     ap.add_argument("--hv-only", action="store_true",
                     help="only H.v on the --lx x --ly lattice with two vectors (6x6: 145 GB); one JSON object")
     ap.add_argument("--chain", type=int, default=0, help="with --hv-only: a periodic chain of this many spins")

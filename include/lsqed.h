@@ -192,7 +192,9 @@ typedef struct lsqed_eps lsqed_eps;
  * basis per step) below 2^16 states, a delayed form (two passes) from 2^16
  * states when ncv <= 32 (LSQED_KS_DELAYED=0/1 forces either); convergence
  * when ||r_i|| / |theta_i| < tol (EPS_CONV_REL).  Allocates (ncv+1)
- * D-vectors of workspace. */
+ * D-vectors of workspace, and with the delayed form one more when the device
+ * keeps 1 GiB free beyond it (the update pass then writes the next residual
+ * to a column it does not read; results are the same either way). */
 int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out);
 int lsqed_eps_solve(lsqed_eps* e, int* nconv, lsqed_stream_t stream);
 /* Solver::solution (src/solver.cpp:58-68): eigenpair i (ascending) and the
