@@ -42,6 +42,13 @@ static const char* kJitBody = R"JIT(
 #ifndef NT_STAGE
 #define NT_STAGE 0
 #endif
+// ONE_ROW (single-model ROWSTAGE, grid of one block per row): the row loop
+// runs once, so that nothing computed before it stays live across rows (the
+// looped form keeps tables and constants in registers and spills 12 VGPRs at
+// four waves per SIMD, 52 bytes per lane of scratch stores per block)
+#ifndef ONE_ROW
+#define ONE_ROW 0
+#endif
 // LOWTAB (with ROWSTAGE): the low bonds of an entry come from a host-built
 // record per low pattern l -- for each flipping low bond (LOWFK order) the
 // partner's index in the row, or RMAX (a zero entry of the staged row) when
@@ -264,7 +271,7 @@ __device__ __forceinline__ void hv_body(const JitArgs& a) {
   const int xcd = blockIdx.x & 7, P = (int)(G >> 3), seg = a.ntasks >> 3;
   double dotacc = 0.0;
   int pb = 0;
-  for (int t = (int)(blockIdx.x >> 3); t < seg; t += P, pb ^= 1) {
+  for (int t = (int)(blockIdx.x >> 3); t < seg; t += (ONE_ROW ? seg : P), pb ^= 1) {
     const XV* __restrict__ xv = (const XV*)a.x;
     XV* __restrict__ yv = (XV*)a.y;
     constexpr double jh_[2] = {JH0, JH1};
@@ -706,6 +713,7 @@ struct JitModule {
   hipFunction_t fn_dot = nullptr, fn_nodot = nullptr;
   bool ok = false;
   bool rowstage = false;  // one block per row (rows_ord), else 256 threads per item
+  bool one_row = false;   // rowstage kernel whose blocks take exactly one row
   int threads = 256;      // block size
   uint64_t src_hash = 0;  // FNV-1a of the generated source
   void* d_lowtab = nullptr;  // LOWTAB records on the module's device (kept for the process)
@@ -721,6 +729,13 @@ struct JitModule {
 static bool jit_rowstage() {
   const char* e = getenv("LSQED_JIT_ROWSTAGE");
   return !(e && atoi(e) == 0);
+}
+
+// Rows a single-model row-staged block takes (LSQED_JIT_ROWS_PER_BLOCK,
+// 1..64; default 1, measured at the grid below).
+static int jit_rows_per_block() {
+  const char* e = getenv("LSQED_JIT_ROWS_PER_BLOCK");
+  return e ? std::max(1, std::min(64, atoi(e))) : 1;
 }
 
 // Threads of a row-staged block: 256 for rows of at most C(14, 7) entries
@@ -898,6 +913,7 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<ui
     const uint64_t rmax = binom_u64(L, L / 2);
     o << "#define ROWSTAGE 1\n#define ROW_THREADS " << jit_row_threads(L) << "\nconstexpr int RMAX = " << rmax
       << ";\n";
+    if (!batch && jit_rows_per_block() == 1) o << "#define ONE_ROW 1\n";
     if (const int mode = jit_lowtab(batch, L)) {
       // mode 1: every flipping low bond in the record; mode 2: only those
       // with a site in each byte of the pattern (lowt 2), the others computed
@@ -980,6 +996,7 @@ static std::shared_ptr<JitModule> jit_compile(const std::string& src, bool batch
   }
   m->ok = true;
   m->rowstage = src.find("#define ROWSTAGE 1") != std::string::npos;
+  m->one_row = m->rowstage && src.find("#define ONE_ROW 1\n") != std::string::npos;
   const size_t rt = src.find("#define ROW_THREADS ");
   m->threads = m->rowstage && rt != std::string::npos ? atoi(src.c_str() + rt + 20) : 256;
   return m;
@@ -1041,13 +1058,14 @@ static int jit_grid_of(lsqed_ham* h, const JitModule* m) {
     // 34.84 ms against 35.67-35.72 with two rows per block, 6x6 625.1 against
     // 636.2 ms, 8x4 29.33 against 30.01 ms.  In h order two rows per block had
     // been better (a block's once-per-block work -- its tables and the stores
-    // of the 12 VGPRs the kernel spills at four waves per SIMD -- shared by
-    // two rows): rows per block on 4x8 / 6x6 1: 39.5 / 701.7 ms, 2: 39.1 /
+    // of the 12 VGPRs the looped kernel spills at four waves per SIMD --
+    // shared by two rows; the one-row form spills nothing: 4x8 34.75 -> 34.20
+    // ms, 6x6 625 -> 589 ms, writes per 4x8 product 8.2 -> 5.0 GB): rows per
+    // block on 4x8 / 6x6 1: 39.5 / 701.7 ms, 2: 39.1 /
     // 688.8 ms, 4: 38.6 / 714.1 ms, 8: 40.8 ms on 4x8 at the time.  A
     // persistent grid (one block per CU dealing rows cyclically) measured 7%
     // slower: a fixed share of rows per CU is unevenly long.
-    const char* e = getenv("LSQED_JIT_ROWS_PER_BLOCK");
-    const int k = e ? std::max(1, std::min(64, atoi(e))) : 1;
+    const int k = m->one_row ? 1 : jit_rows_per_block();
     const int per_xcd = t.nrows_ord / 8;
     return 8 * std::max(1, (per_xcd + k - 1) / k);
   }
