@@ -868,10 +868,12 @@ std::string jit_source(const lsqed_ham* h, bool batch, bool cplx, std::vector<ui
   // N=26 0.199 / 0.227; the batched disorder solve 839 / 810 realisations/s.
   // With the cross gathers as buffer loads: 4x8 37.33 / 36.07 ms (12: 36.68),
   // 6x6 (64-bit ranks) 656.1 / 638.9 ms (12: 661.3; before the buffer loads
-  // 686 / 691).  LSQED_JIT_GATHER_BATCH (2..16) overrides.
+  // 686 / 691).  With the one-row kernel form (no spills at 16): 4x8 12 / 16 /
+  // 20 / 24: 35.40 / 34.88-35.05 / 37.16 / 38.87 ms, 6x6 16 / 20: 595.6 /
+  // 625.3 ms.  LSQED_JIT_GATHER_BATCH (2..32) overrides.
   int gather_batch = (!batch && !cplx && 2 * crq.size() + him.size() >= 110) ? 16 : 8;
   if (const char* w = getenv("LSQED_JIT_GATHER_BATCH"))
-    if (atoi(w) >= 2 && atoi(w) <= 16) gather_batch = atoi(w);
+    if (atoi(w) >= 2 && atoi(w) <= 32) gather_batch = atoi(w);
   o << "#define GATHER_BATCH " << gather_batch << "\n";
   if (const char* w = getenv("LSQED_JIT_MASKED_GATHER"))
     if (atoi(w) == 1) o << "#define MASKED_GATHER 1\n";
