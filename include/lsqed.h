@@ -202,6 +202,11 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv, lsqed_stream_t stream);
  * be NULL; otherwise D doubles on the device (unit norm). */
 int lsqed_eps_solution(lsqed_eps* e, int i, double* lambda, double* x, double* error, lsqed_stream_t stream);
 int lsqed_eps_iterations(const lsqed_eps* e);
+/* Device bytes the solver holds (basis columns and scratch): memory planning
+ * for sectors of several GB per vector, and which basis layout was taken.
+ * No counterpart in the reference (PETSc/SLEPc report memory per object
+ * through their logging). */
+int64_t lsqed_eps_workspace_bytes(const lsqed_eps* e);
 int lsqed_eps_destroy(lsqed_eps* e); /* Solver::SolverClean (src/solver.cpp:28-32) */
 
 /* Plain three-vector Lanczos for the ground energy only (no

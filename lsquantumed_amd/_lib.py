@@ -71,6 +71,7 @@ SIGNATURES = {
     "lsqed_eps_solve": (_i32, [_vp, _P(_i32), _vp]),
     "lsqed_eps_solution": (_i32, [_vp, _i32, _P(_dbl), _vp, _P(_dbl), _vp]),
     "lsqed_eps_iterations": (_i32, [_vp]),
+    "lsqed_eps_workspace_bytes": (_i64, [_vp]),
     "lsqed_eps_destroy": (_i32, [_vp]),
     "lsqed_lanczos_ground": (_i32, [_vp, _i32, _dbl, _u64, _vp, _P(_dbl), _P(_i32), _vp]),
     "lsqed_lanczos_step": (_i32, [_vp, _vp, _dbl, _vp, _dbl, _dbl, _vp, _P(_dbl), _P(_dbl), _vp]),

@@ -1155,6 +1155,12 @@ int lsqed_eps_destroy(lsqed_eps* e) {
 
 int lsqed_eps_iterations(const lsqed_eps* e) { return e ? e->its : 0; }
 
+int64_t lsqed_eps_workspace_bytes(const lsqed_eps* e) {
+  if (!e) return 0;
+  return (int64_t)e->vcols * e->h->D * (int64_t)sizeof(double) +
+         (int64_t)((2 * kMaxNcv + 4) * kRedBlocks + kMaxNcv * kMaxNcv + 2 * kMaxNcv) * (int64_t)sizeof(double);
+}
+
 int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
   if (!e) return fail(LSQED_ERR_ARG, "null solver");
   Ctx c{e, (hipStream_t)stream, e->h->D};

@@ -48,6 +48,10 @@ class EPS:
     def iterations(self) -> int:
         return lib().lsqed_eps_iterations(self.handle)
 
+    def workspace_bytes(self) -> int:
+        """Device bytes the solver holds (basis columns and scratch)."""
+        return int(lib().lsqed_eps_workspace_bytes(self.handle))
+
 
 class Solver:
     @staticmethod

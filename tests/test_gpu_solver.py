@@ -438,6 +438,51 @@ def test_refined_and_delayed_cgs2(O, torch, monkeypatch):
         assert abs(v + 7.142296360616785) <= 1e-12 * 7.2, (v0[0], v1[0])
 
 
+def test_delayed_cgs2_spare_column_changes_nothing(O, torch, monkeypatch):
+    """The delayed CGS2 puts A u in a spare basis column when the device has
+    room, so that its update pass writes the next residual to a column it
+    does not read; LSQED_KS_SPARE=0 keeps the residual over A u.  On the
+    periodic 4x6 J1-J2 sector (nev=4, ncv=20, tol 1e-9) both give the same
+    eigenvalues, errors, iteration counts and eigenvectors to the last bit,
+    the spare form holds exactly one D-vector more (lsqed_eps_workspace_bytes),
+    and the lowest value is the oracle's ground level (committed fixture) to
+    1e-9."""
+    import json
+    import os
+
+    import lsquantumed_amd as Q
+    with open(os.path.join(os.path.dirname(__file__), "golden", "config2_square4x6_j1j2.json")) as f:
+        ref = np.array(json.load(f)["eigenvalues"])
+    env = Q.Environment(24)
+    h = make("square", 24, 0, 1.0, 1.0, 0.5, 0.5, 4, 6)
+    D = h.hamilt_size()
+    runs = {}
+    for spare in ("1", "0"):
+        monkeypatch.setenv("LSQED_KS_SPARE", spare)
+        eps = Q.Solver.SolverInit(h, 4, 20, -2, tol=1e-9)
+        held = eps.workspace_bytes()
+        nconv = Q.Solver.solve(env, eps)
+        assert nconv >= 4
+        vals, errs, vecs = [], [], []
+        for i in range(4):
+            v = torch.empty(D, dtype=torch.float64, device="cuda")
+            lam, err = Q.Solver.solution(eps, i, v)
+            vals.append(lam)
+            errs.append(err)
+            vecs.append(v)
+        runs[spare] = (held, eps.iterations(), np.array(vals), np.array(errs), vecs)
+        Q.Solver.SolverClean(eps)
+        say("spare", spare, "workspace bytes", held, "iterations", runs[spare][1], np.round(vals, 12))
+    a, b = runs["1"], runs["0"]
+    assert a[0] - b[0] == 8 * D and b[0] >= 21 * 8 * D, (a[0], b[0], 8 * D)
+    assert a[1] == b[1]
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3]), (a[2], b[2], a[3], b[3])
+    for x, y in zip(a[4], b[4]):
+        assert torch.equal(x, y)
+    assert max(a[3]) <= 1e-8, a[3]
+    assert abs(a[2][0] - ref[0]) <= 1e-9 * abs(ref[0]), (a[2][0], ref[0])
+
+
 def test_square4x8_ground_state_sum_rules(O, torch):
     """4x8 J1-J2 (D = 601,080,390) ground state from the batched solver (one
     model, vectors kept): E0 matches the three-vector Lanczos to 1e-10; the
