@@ -265,6 +265,43 @@ def test_hv_auto_kernel_is_specialised(torch, lat, N, mag, lx, ly, builds):
     assert h.hv_kernel() == 3
 
 
+@pytest.mark.parametrize("lat,N,mag,lx,ly,builds", [
+    ("square", 24, 0, 4, 6, ("build_diag", "build_off_diag")),
+    ("chain", 22, 0, 0, 0, ("build_diag", "build_off_diag", "build_disorder")),
+    ("square", 32, 2, 4, 8, ("build_diag", "build_off_diag", "build_disorder"))])
+def test_hv_one_row_form_equals_looped_form(O, torch, monkeypatch, lat, N, mag, lx, ly, builds):
+    """With one row per block (the default grid) the specialised kernel is
+    generated with a row loop that runs once; LSQED_JIT_ROWS_PER_BLOCK=2 / 3
+    gives the looped form over a smaller grid.  All three give the same
+    product to the last bit (each entry's sum runs in bond order whatever
+    block takes the row), and the default one matches the oracle's CSR to
+    1e-13 on the smaller sectors."""
+    hi = np.random.default_rng(N).uniform(-1, 1, N) if "build_disorder" in builds else None
+    ys = []
+    x = None
+    for k in (None, "2", "3"):
+        if k is None:
+            monkeypatch.delenv("LSQED_JIT_ROWS_PER_BLOCK", raising=False)
+        else:
+            monkeypatch.setenv("LSQED_JIT_ROWS_PER_BLOCK", k)
+        h = make(lat, N, mag, 1.0, 0.9, 0.5, 0.4, lx, ly, builds=builds, hi=hi)
+        assert h.hv_kernel() == 3
+        if x is None:
+            D = h.hamilt_size()
+            x = torch.from_numpy(np.random.default_rng(D).uniform(-1, 1, D)).cuda()
+        ys.append(h.matvec(x))
+        say("one-row vs looped", lat, N, mag, "rows per block", k or "default")
+    for y in ys[1:]:
+        assert torch.equal(ys[0], y)
+    if h.hamilt_size() <= 3_000_000:
+        m = oracle_model(O, lat, N, mag, 1.0, 0.9, 0.5, 0.4, lx, ly)
+        A = m.csr(hi, flags_of(O, builds))
+        xr = x.cpu().numpy()
+        yref = A @ xr
+        err = np.abs(ys[0].cpu().numpy() - yref).max() / max(1.0, np.abs(yref).max())
+        assert err <= 1e-13, err
+
+
 @pytest.mark.parametrize("lat,N,mag,lx,ly", [("chain", 38, -17, 0, 0), ("chain", 40, -18, 0, 0),
                                               ("chain", 40, 18, 0, 0), ("square", 40, -17, 4, 10)])
 @pytest.mark.parametrize("kernel", [1, 3], ids=["generic", "split_jit"])
