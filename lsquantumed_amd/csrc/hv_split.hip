@@ -624,6 +624,12 @@ int split_tables(lsqed_ham* h, SplitTables& t, bool batch) {
 // at 16); chains with J2 = 0.2: N = 24 0.0614 / 0.0605 / 0.0682, N = 26
 // 0.219 / 0.203 / 0.221, N = 28 0.827 / 0.882 at 13 / 14; N = 18 / 20 / 22
 // at L = 10: 9.7 / 11.6 / 22.3 us against 49 / 94 / 105 us at L = 16.
+// With one row per block and the one-row kernel form, L = 14 / 15 (512-thread
+// blocks at 15): 4x7 1.818-1.832 / 1.834-1.856 ms, 5x6 7.34 / 7.37-7.38, 6x5
+// 6.90 / 7.09-7.10, 8x4 29.10-29.17 / 30.15-30.17, 4x9 648.3 / 681.0, 6x6
+// 589.8 / 623.3, and only 4x8 the other way, 34.19-34.28 / 33.88-33.92 ms (the
+// same torus as 8x4 under another site numbering, which decides how many
+// bonds fall among the L lowest sites), so 14 stays.
 // LSQED_SPLIT_L (8..16, keeping at most 24 high bits) overrides.
 int split_low_bits(const lsqed_ham* h, bool batch) {
   const int N = h->N;
