@@ -591,6 +591,9 @@ def krylov_schur_4x8(tol: float = 1e-9):
     log(f"4x8 Krylov-Schur: {dt:.1f} s, {its} restarts, nconv {nconv}")
     del eps
     torch.cuda.empty_cache()
+    # the solve may have run on a renumbered copy of the sites: prepare the
+    # model's own kernel before the timed Lanczos, which uses it
+    h.hv_kernel()
     r = Q.solver.lanczos_ground_residual(h, tol=1e-11, maxit=800, seed=5)
     e0, steps, dl = r["E0"], r["steps"], r["wall_s"]
     torch.cuda.empty_cache()

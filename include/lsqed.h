@@ -173,7 +173,7 @@ typedef struct {
   int nev, ncv, mpd;
   double tol;
   int maxit;
-  uint64_t seed;  /* start vector: splitmix64 hash of (seed, row), uniform in [-1, 1) */
+  uint64_t seed;  /* start vector: splitmix64 hash of (seed, row of the solver's numbering), uniform in [-1, 1) */
   /* Multiplicity check (no counterpart in SLEPc): once nev values have
    * converged they are locked and the iteration continues from a random
    * vector orthogonal to them until one more value converges, so that copies
@@ -194,7 +194,13 @@ typedef struct lsqed_eps lsqed_eps;
  * when ||r_i|| / |theta_i| < tol (EPS_CONV_REL).  Allocates (ncv+1)
  * D-vectors of workspace, and with the delayed form one more when the device
  * keeps 1 GiB free beyond it (the update pass then writes the next residual
- * to a column it does not read; results are the same either way). */
+ * to a column it does not read; results are the same either way).  On
+ * sectors of 2^24 states or more the solver runs its products on a copy of
+ * the model with the sites renumbered so that the specialised kernel's low
+ * block holds more bonds (4x8 J1-J2: 34.2 -> 29.3 ms per product), when that
+ * gains at least 4 bonds; lsqed_eps_solution returns eigenvectors in the
+ * model's own basis order (LSQED_RELABEL=0 keeps the numbering, =1 applies it
+ * on any sector). */
 int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out);
 int lsqed_eps_solve(lsqed_eps* e, int* nconv, lsqed_stream_t stream);
 /* Solver::solution (src/solver.cpp:58-68): eigenpair i (ascending) and the

@@ -16,8 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblsqed.so")
-SOURCES = ["hamiltonian.hip", "hv_split.hip", "krylov.hip", "observables.hip", "batched.hip", "lattice.cpp",
-           "hv_jit.cpp"]
+SOURCES = ["hamiltonian.hip", "hv_split.hip", "krylov.hip", "observables.hip", "batched.hip", "relabel.hip",
+           "lattice.cpp", "hv_jit.cpp"]
 
 ARCH = os.environ.get("LSQED_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -779,7 +779,15 @@ struct lsqed_eps {
   std::vector<double> evals;
   int nconv = 0, its = 0;
   bool solved = false;
+  // Renumbered copy of the model whose products the solver runs (relabel.hip;
+  // null: h itself), perm[old site] = new site, and perm on the device.
+  lsqed_ham* hk = nullptr;
+  std::vector<int> perm;
+  int* d_perm = nullptr;
 };
+
+// The model the solver's own products use: its renumbered copy when it has one.
+static lsqed_ham* kernel_model(const lsqed_eps* e) { return e->hk ? e->hk : e->h; }
 
 namespace {
 
@@ -1140,6 +1148,27 @@ int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out) {
     lsqed_eps_destroy(e);
     return fail(LSQED_ERR_OOM, "cannot allocate solver scratch");
   }
+  // Products on a renumbered copy of the model where that puts more bonds in
+  // the specialised kernel's low block (relabel.hip); its kernel is compiled
+  // here, not inside the solve.  Kept only when the copy takes the
+  // specialised kernel; otherwise the solver runs on h.
+  std::vector<int> perm;
+  if (relabel_plan(h, perm)) {
+    lsqed_ham* hk = nullptr;
+    const bool ok = relabel_handle(h, perm, &hk) == LSQED_OK && lsqed_ham_get_hv_kernel(hk) == 3 &&
+                    hv_launch_grid(hk) > 0 &&
+                    hipMalloc((void**)&e->d_perm, perm.size() * sizeof(int)) == hipSuccess &&
+                    hipMemcpy(e->d_perm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+      e->hk = hk;
+      e->perm = perm;
+    } else {
+      (void)hipGetLastError();
+      if (hk) lsqed_ham_destroy(hk);
+      if (e->d_perm) (void)hipFree(e->d_perm);
+      e->d_perm = nullptr;
+    }
+  }
   *out = e;
   return LSQED_OK;
 }
@@ -1149,6 +1178,8 @@ int lsqed_eps_destroy(lsqed_eps* e) {
   (void)hipFree(e->V);
   (void)hipFree(e->part);
   (void)hipFree(e->dsm);
+  if (e->hk) lsqed_ham_destroy(e->hk);
+  if (e->d_perm) (void)hipFree(e->d_perm);
   delete e;
   return LSQED_OK;
 }
@@ -1165,6 +1196,10 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
   if (!e) return fail(LSQED_ERR_ARG, "null solver");
   Ctx c{e, (hipStream_t)stream, e->h->D};
   const int m = e->ncv;
+  if (e->hk) {
+    const int rs = relabel_sync(e->h, e->hk, e->perm);  // the model may have changed since create
+    if (rs) return rs;
+  }
   const int64_t D = c.D;
   int rc;
   auto col = [&](int j) { return e->V + (int64_t)j * D; };
@@ -1215,7 +1250,7 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
       double* au = e->vcols >= m + 2 ? col(m + 1) : nullptr;
       for (int j = k; j < m; ++j) {
         double* y = au ? au : col(j + 1);
-        if ((rc = launch_hv(e->h, col(j), y, nullptr, c.s))) return rc;
+        if ((rc = launch_hv(kernel_model(e), col(j), y, nullptr, c.s))) return rc;
         if ((rc = dcgs_project(c, j, col(j), y, q))) return rc;
         const double* cc = q.data();
         const double* hh = q.data() + j;
@@ -1276,7 +1311,7 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
       }
     }
     for (int j = k; j < m && !delayed; ++j) {
-      if ((rc = launch_hv(e->h, col(j), col(j + 1), nullptr, c.s))) return rc;
+      if ((rc = launch_hv(kernel_model(e), col(j), col(j + 1), nullptr, c.s))) return rc;
       std::vector<double> hcoef;
       double b;
       bool lindep = false, scaled = false;
@@ -1371,11 +1406,20 @@ int lsqed_eps_solution(lsqed_eps* e, int i, double* lambda, double* x, double* e
   const double lam = e->evals[i];
   if (lambda) *lambda = lam;
   double* vi = e->V + (int64_t)i * D;
-  if (x) LSQED_HIP(hipMemcpyAsync(x, vi, D * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  if (x && e->hk) {
+    const int rg = relabel_gather(e->h, e->d_perm, vi, x, c.s);  // back to the model's basis order
+    if (rg) return rg;
+  } else if (x) {
+    LSQED_HIP(hipMemcpyAsync(x, vi, D * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  }
+  if (error && e->hk) {
+    const int rs = relabel_sync(e->h, e->hk, e->perm);
+    if (rs) return rs;
+  }
   if (error) {
     // EPSComputeError(EPS_ERROR_RELATIVE): ||H x - lambda x|| / |lambda|
     double* y = e->V + (int64_t)e->ncv * D;
-    int rc = launch_hv(e->h, vi, y, nullptr, c.s);
+    int rc = launch_hv(kernel_model(e), vi, y, nullptr, c.s);
     if (rc) return rc;
     const int G = red_blocks(D);
     hipLaunchKernelGGL(k_resid, dim3(G), dim3(kThreads), 0, c.s, y, vi, lam, D, e->part);

@@ -161,6 +161,15 @@ int split_grid(const lsqed_ham* h);
 int launch_hv_split(lsqed_ham* h, const double* x, double* y, double* dot_part, int grid, hipStream_t s);
 bool split_jit_fits(const lsqed_ham* h, bool batch = false);
 
+// Site renumbering for the device-resident eigensolver (relabel.hip):
+// perm[old site] = new site when it puts more flipping bonds among the row
+// split's low sites; the renumbered copy of a model, its synchronisation with
+// the model, and the gather of a renumbered vector back into h's basis order.
+bool relabel_plan(const lsqed_ham* h, std::vector<int>& perm);
+int relabel_handle(const lsqed_ham* h, const std::vector<int>& perm, lsqed_ham** out);
+int relabel_sync(const lsqed_ham* h, lsqed_ham* hk, const std::vector<int>& perm);
+int relabel_gather(const lsqed_ham* h, const int* d_perm, const double* src, double* dst, hipStream_t s);
+
 // Specialised row-split kernel (hv_jit.cpp).
 bool jit_enabled();
 int jit_prepare(lsqed_ham* h);

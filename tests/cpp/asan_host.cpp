@@ -122,7 +122,68 @@ static int check_records(int lat, int N, int lx, int ly, bool batch, double J2, 
   return 1;
 }
 
+// Site renumbering plans: a bijection of the sites that puts at least 4 more
+// flipping bonds among the row split's low sites, none where the numbering
+// is already best; the 4x8 torus reaches the 43 low bonds that its 8x4
+// numbering has.
+static int low_bonds(const lsqed_ham& h, const std::vector<int>& perm, int L) {
+  int n = 0;
+  for (int k = 0; k < 2; ++k)
+    for (size_t b = 0; b < h.bi[k].size(); ++b) {
+      const int i = perm.empty() ? h.bi[k][b] : perm[h.bi[k][b]], j = perm.empty() ? h.bj[k][b] : perm[h.bj[k][b]];
+      n += h.J[k] != 0.0 && i != j && i < L && j < L;
+    }
+  return n;
+}
+
+static int check_relabel(int lat, int N, int lx, int ly, double J2, int expect_low) {
+  lsqed_ham h;
+  h.N = N;
+  h.nup = N / 2;
+  h.D = lsqed_sector_dim(N, N / 2);
+  h.J[0] = 1.0;
+  h.J[1] = J2;
+  h.Dz[0] = 1.0;
+  h.Dz[1] = J2;
+  EXPECT(bonds(lat, N, lx, ly, h));
+  std::vector<int> perm;
+  const bool on = lsqed::relabel_plan(&h, perm);
+  const int L = lsqed::split_low_bits(&h, false);
+  if (expect_low == -1) {
+    EXPECT(!on);
+    return 0;
+  }
+  if (expect_low >= 0) EXPECT(on);
+  if (!on) return 0;
+  std::vector<int> seen(N, 0);
+  for (int v : perm) {
+    EXPECT(v >= 0 && v < N);
+    if (v >= 0 && v < N) ++seen[v];
+  }
+  for (int v = 0; v < N; ++v) EXPECT(seen[v] == 1);
+  const int before = low_bonds(h, {}, L), after = low_bonds(h, perm, L);
+  EXPECT(after >= before + 4);
+  EXPECT(after >= expect_low);
+  return 1;
+}
+
 int main() {
+  setenv("LSQED_RELABEL", "1", 1);
+  int nrel = check_relabel(LSQED_SQUARE2D, 32, 4, 8, 0.5, 43);
+  nrel += check_relabel(LSQED_SQUARE2D, 36, 4, 9, 0.5, 43);
+  nrel += check_relabel(LSQED_SQUARE2D, 24, 4, 6, 0.5, 43);
+  nrel += check_relabel(LSQED_SQUARE2D, 24, 2, 12, 0.5, 50);
+  nrel += check_relabel(LSQED_SQUARE2D, 32, 8, 4, 0.5, -1);     // numbering already best
+  nrel += check_relabel(LSQED_SQUARE2D, 36, 6, 6, 0.5, -1);
+  nrel += check_relabel(LSQED_CHAIN1D, 30, 0, 0, 0.2, -1);
+  const int hcr = check_relabel(LSQED_HONEYCOMB2D, 32, 4, 8, 0.4, -2);  // either way; a valid plan if any
+  unsetenv("LSQED_RELABEL");
+  nrel += check_relabel(LSQED_SQUARE2D, 24, 4, 6, 0.5, -1);     // below 2^24 states by default
+  setenv("LSQED_RELABEL", "0", 1);
+  nrel += check_relabel(LSQED_SQUARE2D, 32, 4, 8, 0.5, -1);
+  unsetenv("LSQED_RELABEL");
+  EXPECT(nrel == 4);
+  std::printf("renumbering plans checked (honeycomb 4x8: %s)\n", hcr ? "renumbered" : "kept");
   int nrec = check_records(LSQED_CHAIN1D, 22, 0, 0, true, 0.2, 2);  // the batched kernel's default
   nrec += check_records(LSQED_CHAIN1D, 18, 0, 0, true, 0.0, 2);      // NNN bonds do not flip, still counted
   setenv("LSQED_JIT_LOWTAB", "1", 1);

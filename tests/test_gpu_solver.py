@@ -483,6 +483,58 @@ def test_delayed_cgs2_spare_column_changes_nothing(O, torch, monkeypatch):
     assert abs(a[2][0] - ref[0]) <= 1e-9 * abs(ref[0]), (a[2][0], ref[0])
 
 
+@pytest.mark.parametrize("fields", [False, True], ids=["no_fields", "fields"])
+def test_krylov_schur_on_renumbered_sites(O, torch, monkeypatch, fields):
+    """The solver may run its products on a copy of the model with the sites
+    renumbered so that the specialised kernel's low block holds more bonds
+    (by default on sectors of 2^24 states or more; LSQED_RELABEL=1 forces it
+    here on the periodic 4x6 J1-J2 sector, whose numbering leaves 37 flipping
+    bonds among the 14 low sites against 43 renumbered).  Against the solve
+    in the model's numbering (LSQED_RELABEL=0), nev=4 ncv=20 tol 1e-12: the
+    same eigenvalues to 1e-11, each returned eigenvector an eigenvector of
+    the model itself in its own basis order (||H x - lambda x|| / |lambda|
+    <= 1e-9 with the model's H.v: a wrong back-permutation fails this), the
+    ground states equal up to sign to 1e-8 but not bit for bit (the two runs
+    take different numberings), and without fields the lowest value the
+    oracle's (committed fixture) to 1e-10."""
+    import json
+    import os
+
+    import lsquantumed_amd as Q
+    with open(os.path.join(os.path.dirname(__file__), "golden", "config2_square4x6_j1j2.json")) as f:
+        ref = np.array(json.load(f)["eigenvalues"])
+    env = Q.Environment(24)
+    builds = ("build_diag", "build_off_diag") + (("build_disorder",) if fields else ())
+    hi = np.random.default_rng(7).uniform(-0.5, 0.5, 24) if fields else None
+    h = make("square", 24, 0, 1.0, 1.0, 0.5, 0.5, 4, 6, builds=builds, hi=hi)
+    D = h.hamilt_size()
+    runs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("LSQED_RELABEL", mode)
+        eps = Q.Solver.SolverInit(h, 4, 20, -2, tol=1e-12)
+        assert Q.Solver.solve(env, eps) >= 4
+        vals, vecs = [], []
+        for i in range(4):
+            v = torch.empty(D, dtype=torch.float64, device="cuda")
+            lam, err = Q.Solver.solution(eps, i, v)
+            assert err <= 1e-10, (mode, i, err)
+            y = h.matvec(v)
+            res = (torch.linalg.norm(y - lam * v) / abs(lam)).item()
+            assert res <= 1e-9, (mode, i, res)
+            vals.append(lam)
+            vecs.append(v)
+        say("renumbered" if mode == "1" else "model numbering", "iterations", eps.iterations(), np.round(vals, 12))
+        Q.Solver.SolverClean(eps)
+        runs[mode] = (np.array(vals), vecs)
+    a, b = runs["1"], runs["0"]
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-11, atol=0)
+    ov = abs(torch.dot(a[1][0], b[1][0]).item())
+    assert abs(ov - 1.0) <= 1e-8, ov
+    assert not torch.equal(a[1][0], b[1][0]) and not torch.equal(a[1][0], -b[1][0])
+    if not fields:
+        assert abs(a[0][0] - ref[0]) <= 1e-10 * abs(ref[0]), (a[0][0], ref[0])
+
+
 def test_square4x8_ground_state_sum_rules(O, torch):
     """4x8 J1-J2 (D = 601,080,390) ground state from the batched solver (one
     model, vectors kept): E0 matches the three-vector Lanczos to 1e-10; the
