@@ -173,7 +173,7 @@ typedef struct {
   int nev, ncv, mpd;
   double tol;
   int maxit;
-  uint64_t seed;  /* start vector: splitmix64 hash of (seed, row of the solver's numbering), uniform in [-1, 1) */
+  uint64_t seed;  /* start vector: splitmix64 hash of (seed, row), uniform in [-1, 1) */
   /* Multiplicity check (no counterpart in SLEPc): once nev values have
    * converged they are locked and the iteration continues from a random
    * vector orthogonal to them until one more value converges, so that copies

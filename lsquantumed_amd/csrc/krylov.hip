@@ -780,7 +780,8 @@ struct lsqed_eps {
   int nconv = 0, its = 0;
   bool solved = false;
   // Renumbered copy of the model whose products the solver runs (relabel.hip;
-  // null: h itself), perm[old site] = new site, and perm on the device.
+  // null: h itself), perm[old site] = new site, and on the device perm then
+  // its inverse (N entries each).
   lsqed_ham* hk = nullptr;
   std::vector<int> perm;
   int* d_perm = nullptr;
@@ -1155,10 +1156,13 @@ int lsqed_eps_create(lsqed_ham* h, const lsqed_eps_opts* o, lsqed_eps** out) {
   std::vector<int> perm;
   if (relabel_plan(h, perm)) {
     lsqed_ham* hk = nullptr;
+    std::vector<int> both(perm);
+    both.resize(2 * perm.size());
+    for (size_t i = 0; i < perm.size(); ++i) both[perm.size() + perm[i]] = (int)i;
     const bool ok = relabel_handle(h, perm, &hk) == LSQED_OK && lsqed_ham_get_hv_kernel(hk) == 3 &&
                     hv_launch_grid(hk) > 0 &&
-                    hipMalloc((void**)&e->d_perm, perm.size() * sizeof(int)) == hipSuccess &&
-                    hipMemcpy(e->d_perm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+                    hipMalloc((void**)&e->d_perm, both.size() * sizeof(int)) == hipSuccess &&
+                    hipMemcpy(e->d_perm, both.data(), both.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
     if (ok) {
       e->hk = hk;
       e->perm = perm;
@@ -1204,9 +1208,18 @@ int lsqed_eps_solve(lsqed_eps* e, int* nconv_out, lsqed_stream_t stream) {
   int rc;
   auto col = [&](int j) { return e->V + (int64_t)j * D; };
 
-  // start vector (SLEPc draws a random one; ours is a seeded hash)
-  hipLaunchKernelGGL(k_fill_random, dim3(stream_grid(D)), dim3(kThreads), 0, c.s, col(0), D, e->seed);
-  LSQED_HIP(hipGetLastError());
+  // start vector (SLEPc draws a random one; ours is a seeded hash), drawn in
+  // the model's basis order also when the products run renumbered (then
+  // gathered into the renumbered order), so that the solve starts from the
+  // same vector either way
+  if (e->hk && e->vcols >= 2) {
+    hipLaunchKernelGGL(k_fill_random, dim3(stream_grid(D)), dim3(kThreads), 0, c.s, col(1), D, e->seed);
+    LSQED_HIP(hipGetLastError());
+    if ((rc = relabel_gather(e->h, e->d_perm + e->h->N, col(1), col(0), c.s))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_fill_random, dim3(stream_grid(D)), dim3(kThreads), 0, c.s, col(0), D, e->seed);
+    LSQED_HIP(hipGetLastError());
+  }
   double nrm;
   if ((rc = norm2(c, col(0), nrm))) return rc;
   if ((rc = scale(c, col(0), 1.0 / nrm))) return rc;

@@ -205,7 +205,8 @@ int relabel_handle(const lsqed_ham* h, const std::vector<int>& perm, lsqed_ham**
 }
 
 // dst (h's basis order) from src (the renumbered basis): dst[r] =
-// src[rank(pi(unrank(r)))], d_perm the N entries of perm on the device.
+// src[rank(pi(unrank(r)))], d_perm the N entries of perm on the device; with
+// the inverse permutation, the other way round.
 int relabel_gather(const lsqed_ham* h, const int* d_perm, const double* src, double* dst, hipStream_t s) {
   if (h->D == 0) return LSQED_OK;
   const RowParams p = make_row_params(h);
