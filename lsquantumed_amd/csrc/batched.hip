@@ -873,6 +873,7 @@ bool batch_nt(const lsqed_batch* b) {
 // new vector overwrites p.  Scalars of the step go to history slot j.
 int enqueue_step(lsqed_batch* b, double* q, double* p, int j, bool accumulate, hipStream_t s) {
   const int np = b->nprob;
+  if (j < 0 || j >= kSteps) return fail(LSQED_ERR_STATE, "batched Lanczos: history slot out of range");
   const int rc = launch_hv_batch(b, q, b->d_y, s);
   if (rc) return rc;
   if (accumulate)
@@ -928,6 +929,7 @@ void drop_graphs(lsqed_batch* b) {
 // [t0, t0 + nt) (problem indices inside the kernels stay absolute).
 int enqueue_group_step(lsqed_batch* b, const Group& g, double* q, double* p, int j, bool accumulate,
                        hipStream_t st) {
+  if (j < 0 || j >= kSteps) return fail(LSQED_ERR_STATE, "batched Lanczos: history slot out of range");
   int64_t work = 0;
   for (const Group& gg : b->groups) work += (int64_t)gg.ntasks * (int64_t)gg.idx.size();
   BatchJitLaunch L{q,           b->d_y,      b->d_hvpart + g.part0, b->d_poff, b->d_pcoup, b->d_pfields,
@@ -1098,8 +1100,11 @@ int time_shape(lsqed_batch* b, float* ms) {
   LSQED_HIP(hipEventCreate(&e1));
   if ((rc = enqueue_step(b, b->d_q, b->d_p, 0, false, b->gs))) return rc;
   LSQED_HIP(hipEventRecord(e0, b->gs));
+  // the history holds kSteps slots (d_hist: 2 kSteps rows of nprob), which
+  // the timed steps reuse cyclically
   for (int j = 1; j <= 4; ++j)
-    if ((rc = enqueue_step(b, (j & 1) ? b->d_p : b->d_q, (j & 1) ? b->d_q : b->d_p, j, false, b->gs))) return rc;
+    if ((rc = enqueue_step(b, (j & 1) ? b->d_p : b->d_q, (j & 1) ? b->d_q : b->d_p, j % kSteps, false, b->gs)))
+      return rc;
   LSQED_HIP(hipEventRecord(e1, b->gs));
   LSQED_HIP(hipGetLastError());
   LSQED_HIP(hipEventSynchronize(e1));
