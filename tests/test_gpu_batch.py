@@ -433,7 +433,7 @@ def test_batch_mixed_lattices_and_sectors(O, torch):
 
 
 def test_disorder_average_sqp_chunked(O, torch, tmp_path):
-    """The driver with Sqp (S^+_i from Sz = 0 into Sz = 1, the target sector
+    """disorder_average with Sqp (S^+_i from Sz = 0 into Sz = 1, the target sector
     carrying the same fields) and batches smaller than the realisation count:
     energies and multiplet-summed weights of every file against the oracle."""
     import lsquantumed_amd as Q
