@@ -299,6 +299,44 @@ def stream_ceiling(q, p, y, reps: int = 10):
             "torch_triad_gbs": round(24 * n / ms_tt / 1e6, 1), "elements": n}
 
 
+def stream_sizes(reps: int = 20):
+    """Read bandwidth of the library's read-only stream (lsqed_stream_probe
+    kind 1) against the size of the buffer it streams, 16 MB to 4 GB: three
+    warm-up passes, then `reps` passes over the same buffer timed with HIP
+    events.  Buffers the memory-side cache (256 MB) or the L2s (8 x 4 MB)
+    hold stream from there on repeated passes."""
+    import ctypes as C
+
+    import torch
+
+    from lsquantumed_amd._lib import check, lib
+
+    part = torch.empty(2048, dtype=torch.float64, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    for mb in (16, 32, 64, 128, 192, 256, 384, 512, 1024, 4096):
+        n = mb * (1 << 20) // 8
+        b = torch.rand(n, dtype=torch.float64, device="cuda")
+
+        def run():
+            check(lib().lsqed_stream_probe(1, None, C.c_void_p(b.data_ptr()), None, n,
+                                           C.c_void_p(part.data_ptr()), st))
+
+        for _ in range(3):
+            run()
+        ev0.record()
+        for _ in range(reps):
+            run()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        out[f"{mb} MB"] = round(8 * n / ms / 1e9, 2)  # TB/s
+        del b
+    torch.cuda.empty_cache()
+    return {"unit": "TB/s", "passes": reps, "read_tbs": out}
+
+
 def calibrate(reps: int = 2):
     """Kernels of known traffic for reading the HBM byte counters (rocprofv3
     --pmc): the read-only stream (8 B per element read), the STREAM triad
@@ -734,6 +772,8 @@ def main():
     ap.add_argument("--hv-only", action="store_true",
                     help="only H.v on the --lx x --ly lattice with two vectors (6x6: 145 GB); one JSON object")
     ap.add_argument("--chain", type=int, default=0, help="with --hv-only: a periodic chain of this many spins")
+    ap.add_argument("--stream-sizes", action="store_true",
+                    help="only the read-stream bandwidth against the streamed buffer's size (16 MB - 4 GB)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only the kernels of known traffic used to read the HBM byte counters (for rocprofv3 --pmc)")
     ap.add_argument("--no-large", action="store_true",
@@ -758,6 +798,9 @@ def main():
     torch.cuda.set_device(0)
     if args.calibrate:
         print(json.dumps({"calibrate": calibrate()}))
+        return
+    if args.stream_sizes:
+        print(json.dumps({"stream_sizes": stream_sizes()}))
         return
     if args.batch_only:
         print(json.dumps({"batched_disorder": disorder_batch()}))
