@@ -1095,25 +1095,30 @@ int time_shape(lsqed_batch* b, float* ms) {
   int rc = upload_problems(b);
   if (rc) return rc;
   if ((rc = start(b, b->gs))) return rc;
-  hipEvent_t e0, e1;
-  LSQED_HIP(hipEventCreate(&e0));
-  LSQED_HIP(hipEventCreate(&e1));
-  if ((rc = enqueue_step(b, b->d_q, b->d_p, 0, false, b->gs))) return rc;
-  LSQED_HIP(hipEventRecord(e0, b->gs));
-  // the history holds kSteps slots (d_hist: 2 kSteps rows of nprob), which
-  // the timed steps reuse cyclically
-  for (int j = 1; j <= 4; ++j)
-    if ((rc = enqueue_step(b, (j & 1) ? b->d_p : b->d_q, (j & 1) ? b->d_q : b->d_p, j % kSteps, false, b->gs)))
-      return rc;
-  LSQED_HIP(hipEventRecord(e1, b->gs));
-  LSQED_HIP(hipGetLastError());
-  LSQED_HIP(hipEventSynchronize(e1));
-  float t = 0.0f;
-  LSQED_HIP(hipEventElapsedTime(&t, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *ms = t / 4.0f;
-  return LSQED_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto timed = [&]() -> int {
+    LSQED_HIP(hipEventCreate(&e0));
+    LSQED_HIP(hipEventCreate(&e1));
+    int r = enqueue_step(b, b->d_q, b->d_p, 0, false, b->gs);
+    if (r) return r;
+    LSQED_HIP(hipEventRecord(e0, b->gs));
+    // the history holds kSteps slots (d_hist: 2 kSteps rows of nprob),
+    // which the timed steps reuse cyclically
+    for (int j = 1; j <= 4; ++j)
+      if ((r = enqueue_step(b, (j & 1) ? b->d_p : b->d_q, (j & 1) ? b->d_q : b->d_p, j % kSteps, false, b->gs)))
+        return r;
+    LSQED_HIP(hipEventRecord(e1, b->gs));
+    LSQED_HIP(hipGetLastError());
+    LSQED_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    LSQED_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / 4.0f;
+    return LSQED_OK;
+  };
+  rc = timed();
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
 }
 
 // Autotune, timed on the batch itself and cached per batch shape:
